@@ -1,0 +1,155 @@
+/* ddx: exact double-dummy solver for the AMD Instinct MI355X (gfx950) -- C ABI.
+ *
+ * The reference (hsperr/dds_gpu) exposes its solver as a header-only C++ API and three
+ * command lines; it has no FFI. These entry points are what a binding would need to replace
+ * that surface: plain pointers and sizes, status codes, no C++ or torch types. The library
+ * never calls exit(); every failure returns a negative status and leaves a message for
+ * ddx_last_error(). Host buffers belong to the caller; device TT and lane state belong to
+ * the handle that allocated them. A handle is used by one host thread at a time.
+ *
+ * Encoding (reference src/dd.h:13-14): card = suit*16 + rank as a bit of a uint64_t hand;
+ * suits 0..3 = S,H,D,C; ranks 0..12 = 2..A; seats 0..3 = N,E,S,W; strains 0..4 =
+ * C,D,H,S,NT. A table is 20 uint8 tricks [strain][declarer N,E,S,W]; declarer E/W values are
+ * 13 - NS tricks, also for endings with fewer than 13 cards (reference src/dd.h:922,937).
+ */
+#ifndef DDX_H
+#define DDX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DDX_OK = 0,
+  DDX_E_ARG = -1,     /* invalid argument (sizes, TT_LOG2 < 9, bad deal) */
+  DDX_E_NOMEM = -2,   /* host or device allocation failed */
+  DDX_E_HIP = -3,     /* HIP runtime error */
+  DDX_E_NODEV = -4,   /* no GPU, or the GPU code is missing */
+  DDX_E_STOPPED = -5, /* ddx_gpu_request_stop() ended the run; unsolved results are 0xFF */
+  DDX_E_IO = -6       /* file format or I/O error */
+};
+
+/* Seed policies for the trick-count search of each lead. */
+enum {
+  DDX_SEED_PLAIN = 0, /* binary search, 7 then the previous lead's answer (dd_solve_strain) */
+  DDX_SEED_STEP = 1,  /* step from seeds[] one trick at a time (dd_solve_strain_guess) */
+  DDX_SEED_CERT = 2,  /* probe seed, then seed +- 1, then binary search (learned seeds) */
+  DDX_SEED_CERTP = 3  /* DDX_SEED_CERT seeded with the previous lead's answer (no seeds[]) */
+};
+
+/* Counters of one solve call. */
+typedef struct ddx_stats {
+  uint64_t nodes;        /* search nodes entered (reference DDCtx::nodes) */
+  uint64_t tt_probes;    /* trick-start TT probes (reference DDCtx::st_probe) */
+  uint64_t tt_hits;      /* probes that decided the node (reference DDCtx::st_hit) */
+  uint64_t jobs_done;    /* jobs finished */
+  double seconds;        /* wall time of the solve, device work included */
+} ddx_stats;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* ddx_last_error(void);
+
+/* Library version string. */
+const char* ddx_version(void);
+
+/* ---- deals and files ------------------------------------------------------------------ */
+
+/* Deals [first, first + n) of the seeded stream `seed`, k cards per hand (1..13), into
+ * hands[4*n] (N,E,S,W). Replaces the reference's random_deals / random_deal
+ * (gpu/gpu_main.cu:154-163, gpu/stream_main.cu:116-122), whose std::shuffle is not portable. */
+int ddx_random_deals(uint64_t seed, int64_t first, int64_t n, int k, uint64_t* hands);
+
+/* Rows [start, start + n) of a Pgx DDS-table .npy file: hands[4*n] and, when tables is not
+ * NULL, the stored tables[20*n]. n < 0 reads to the end; *n_read gets the count.
+ * Replaces pgx_load / pgx_hands / pgx_tricks (src/npy.h:15-63). */
+int ddx_pgx_read(const char* path, int64_t start, int64_t n, uint64_t* hands, uint8_t* tables,
+                 int64_t* n_read);
+/* Number of rows of a Pgx file. */
+int ddx_pgx_rows(const char* path, int64_t* rows);
+/* Writes deals and tables (tables may be NULL: zeros) as a Pgx file. */
+int ddx_pgx_write(const char* path, const uint64_t* hands, const uint8_t* tables, int64_t n);
+
+/* ---- CPU solver ------------------------------------------------------------------------ */
+
+/* One CPU solver context with its own TT of 2^tt_log2 buckets (tt_log2 >= 9), like the
+ * reference's caller-owned DDCtx + TT (cpu/cpu_main.cpp:43-51). Single-threaded. */
+typedef struct ddx_cpu ddx_cpu;
+int ddx_cpu_create(int tt_log2, ddx_cpu** out);
+void ddx_cpu_destroy(ddx_cpu* ctx);
+
+/* Tricks for declarers N,E,S,W in one strain; the four leads share the TT
+ * (reference dd_solve_strain, src/dd.h:926-939). */
+int ddx_cpu_solve_strain(ddx_cpu* ctx, const uint64_t hands[4], int strain, int out4[4]);
+/* Same, stepping from a guess of NS tricks per declarer
+ * (reference dd_solve_strain_guess, src/dd.h:915-924). */
+int ddx_cpu_solve_strain_guess(ddx_cpu* ctx, const uint64_t hands[4], int strain,
+                               const int ns4[4], int out4[4]);
+/* NS tricks with `leader` on lead, binary search from `guess`, in the strain and deal of the
+ * last solve_strain call (reference dd_ns_tricks, src/dd.h:877-891). */
+int ddx_cpu_ns_tricks(ddx_cpu* ctx, const uint64_t hands[4], int strain, int leader, int guess,
+                      int* ns);
+/* Search nodes entered by this context so far (reference DDCtx::nodes). */
+uint64_t ddx_cpu_nodes(const ddx_cpu* ctx);
+
+/* Jobs of n deals on `threads` host threads, each with its own TT of 2^tt_log2 buckets and
+ * an atomic job counter (reference dd_cpu, cpu/cpu_main.cpp:42-74). jobs_per_deal is 5 (one
+ * job per strain, leads share the TT) or 20 (one per lead). seeds: n*20 NS-trick seeds for
+ * DDX_SEED_STEP / DDX_SEED_CERT, else NULL. With threads == 1 the jobs run in index order
+ * on one context and job_nodes (if not NULL, n*jobs_per_deal entries) gets the nodes of
+ * each job. */
+int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads, int tt_log2,
+                        int jobs_per_deal, int seed_mode, const uint8_t* seeds, uint8_t* out,
+                        uint64_t* job_nodes, ddx_stats* stats);
+
+/* ---- GPU solver (MI355X) --------------------------------------------------------------- */
+
+typedef struct ddx_gpu_config {
+  int device;           /* HIP device ordinal */
+  int64_t lanes;        /* concurrent solver lanes, each with its own TT (default 65536) */
+  int tt_log2;          /* buckets per lane TT, >= 9 (default 10) */
+  int jobs_per_deal;    /* 5 or 20 */
+  int schedule;         /* stage schedule of the wavefront loop, see DDX_SCHED_* */
+  int seed_mode;        /* DDX_SEED_* */
+  int block;            /* threads per block (multiple of 64; default 256) */
+  int profile;          /* 1: collect per-stage counters (slower) */
+} ddx_gpu_config;
+
+enum {
+  DDX_SCHED_PRIORITY = 0, /* one stage per round: RET > NEXT > ROOT > JOB > ENTER (reference) */
+  DDX_SCHED_CHAIN = 1,    /* cheap stages run back to back per lane, then ENTER for all */
+  DDX_SCHED_SPLIT = 2     /* as CHAIN, and follow nodes enter before trick-start nodes */
+};
+
+typedef struct ddx_gpu ddx_gpu;
+
+/* Default configuration (device 0, 65536 lanes, TT_LOG2 10, 5 jobs per deal). */
+void ddx_gpu_default_config(ddx_gpu_config* cfg);
+/* Allocates the lanes' TTs and stacks on the device; replaces the reference's per-run
+ * cudaMalloc of TT and DDWave state (gpu/gpu_main.cu:205-252). */
+int ddx_gpu_open(const ddx_gpu_config* cfg, ddx_gpu** out);
+void ddx_gpu_close(ddx_gpu* h);
+/* Device bytes held by the handle. */
+uint64_t ddx_gpu_device_bytes(const ddx_gpu* h);
+
+/* Solves n deals: host hands[4n] in, host out[20n] (0xFF = unsolved after a stop). seeds:
+ * host n*20 NS-trick seeds for DDX_SEED_STEP/CERT, else NULL. Replaces the reference's
+ * wave_kernel batch run (gpu/gpu_main.cu:61-152, 231-274). */
+int ddx_gpu_solve(ddx_gpu* h, const uint64_t* hands, int64_t n, const uint8_t* seeds,
+                  uint8_t* out, ddx_stats* stats);
+/* Same with device-resident buffers (HIP device pointers), on HIP stream `stream` (NULL =
+ * the null stream); returns after the work is queued. Then ddx_gpu_wait(). */
+int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t n, const uint8_t* d_seeds,
+                        uint8_t* d_out, void* stream);
+/* Waits for the last async solve and fills stats. */
+int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats);
+/* Asks a running solve to stop soon (reference TIME_LIMIT stop flag, gpu/gpu_main.cu:90,264);
+ * callable from another thread. */
+void ddx_gpu_request_stop(ddx_gpu* h);
+/* Duration in ms of the last solve kernel, from HIP events on its stream. */
+double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDX_H */

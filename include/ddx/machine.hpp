@@ -1,0 +1,246 @@
+// ddx core, part 5: the resumable solver machine shared by the CPU and the GPU.
+//
+// A lane runs jobs; a job is one (deal, strain) -- its four leads share one TT generation --
+// or one (deal, strain, declarer) when jobs_per_deal is 20 (reference src/dd_wave.h:17-147).
+// Each lead's NS trick count comes from a sequence of null-window searches; how the next
+// target is chosen is the seed policy:
+//   PLAIN  binary search over [0, 13], first target 7 then the previous lead's answer
+//          (reference dd_ns_tricks / dd_solve_strain, src/dd.h:877-891, 926-939)
+//   STEP   from a given seed, step up or down one trick at a time
+//          (reference dd_ns_tricks_from / dd_solve_strain_guess, src/dd.h:898-924)
+//   CERT   probe the seed g and then g+1 (or g-1): a right seed costs two searches, the
+//          certificate "at least g, not g+1"; after that, binary search on what is left.
+//          Seeds come from the caller (e.g. the trick-count net).
+//   CERTP  CERT seeded with the previous lead's answer (7 for the first); the no-net
+//          ablation of CERT.
+// Every policy keeps the invariant answer in [lo, hi] and stops at lo == hi, so all of them
+// return the exact trick count; seeds only change how many searches it takes.
+//
+// Stages: JOB (take a job), ROOT (start a lead), ENTER (enter a node), NEXT (play the next
+// move of the top frame), RET (a child returned), EXIT.
+#pragma once
+#include "search.hpp"
+
+namespace ddx {
+
+enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_COUNT };
+enum { SEED_PLAIN = 0, SEED_STEP = 1, SEED_CERT = 2, SEED_CERTP = 3 };
+
+// The batch a lane works on. Deal d's hands are hands[4d..4d+3]; its results go to
+// out[20d + 4*strain + declarer] as the declarer's tricks (NS count for N/S declarers,
+// 13 - NS count for E/W, also for endings, as the reference does: src/dd.h:922,937).
+struct Batch {
+  const u64* hands;
+  uint8_t* out;
+  const uint8_t* seeds;  // NS-trick seeds in the same layout as out, or null
+  long long jobs;        // deals * jobs_per_deal
+  int jobs_per_deal;     // 5 (per strain) or 20 (per lead)
+  int seed_mode;
+};
+
+struct Lane {
+  Ctx c;
+  Pos p;
+  Frame top;  // frame of the deepest expanded node, when depth > 0
+  int depth;  // expanded nodes on the current path; frames 0..depth-2 are on the stack
+  int stage;
+  int target, lo, hi, searches;
+  int v;  // result of the node that just finished
+  u64 r;  // its relevant cards
+  long long deal;
+  int strain, decl, last_decl, prev_ns, last_true;
+  int exit_at;  // where the last enter() decided (profiling)
+};
+
+DDX_FN void lane_init(Lane& L, TTView tt) {
+  L.c.tt = tt;
+  L.c.gen = 0;
+  L.c.nodes = L.c.probes = L.c.hits = 0;
+  L.depth = 0;
+  L.stage = ST_JOB;
+  L.v = 0;
+  L.r = 0;
+  L.deal = -1;
+  L.strain = L.decl = L.last_decl = 0;
+  L.prev_ns = 7;
+  L.last_true = 0;
+  L.target = L.lo = L.hi = L.searches = 0;
+  L.exit_at = 0;
+}
+
+DDX_FN void start_search(Lane& L) {
+  L.p.leader = (L.decl + 1) & 3;
+  L.p.nplayed = 0;
+  L.p.ns_won = 0;
+  L.p.left = popc(L.p.hand[0]);
+  L.p.trick = 0;
+  L.depth = 0;
+  L.stage = ST_ENTER;
+}
+
+// Loads job `job` of the batch into the lane (reference dd_wave_step JOB, src/dd_wave.h:72-90).
+DDX_FN void lane_load_job(Lane& L, const Batch& B, long long job) {
+  const int jpd = B.jobs_per_deal;
+  L.deal = job / jpd;
+  const int sub = (int)(job % jpd);
+  const u64* h = B.hands + 4 * L.deal;
+#pragma unroll
+  for (int i = 0; i < 4; i++) L.p.hand[i] = h[i];
+  L.strain = jpd == 5 ? sub : sub / 4;
+  L.decl = jpd == 5 ? 0 : sub % 4;
+  L.last_decl = jpd == 5 ? 3 : L.decl;
+  L.p.trump = strain_trump(L.strain);
+  next_generation(L.c);
+  L.prev_ns = 7;
+  L.stage = ST_ROOT;
+}
+
+// ROOT: first target of this lead's search sequence.
+DDX_FN void lane_root(Lane& L, const Batch& B) {
+  const int left = popc(L.p.hand[0]);
+  L.lo = 0;
+  L.searches = 0;
+  int g;
+  if (B.seed_mode == SEED_PLAIN) {
+    L.hi = 13;
+    g = L.prev_ns;
+    L.target = g < 1 ? 1 : g > 13 ? 13 : g;
+  } else {
+    L.hi = left;
+    if (B.seed_mode == SEED_CERTP) g = L.prev_ns;
+    else g = B.seeds ? (int)B.seeds[20 * L.deal + 4 * L.strain + L.decl] : 7;
+    g = g < 0 ? 0 : g > left ? left : g;
+    L.target = g == 0 ? 1 : g;  // "at least 0" holds without a search
+  }
+  start_search(L);
+}
+
+// The root search finished with L.v: narrow [lo, hi]; pick the next target, or record the
+// answer and move to the next lead or job (reference dd_wave_node_done, src/dd_wave.h:42-65).
+DDX_FN void lane_search_done(Lane& L, const Batch& B) {
+  L.searches++;
+  if (L.v) L.lo = L.target;
+  else L.hi = L.target - 1;
+  if (L.lo < L.hi) {
+    const bool adjacent = B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2);
+    if (adjacent) L.target = L.v ? L.lo + 1 : L.hi;
+    else L.target = (L.lo + L.hi + 1) / 2;
+    start_search(L);
+    return;
+  }
+  const int ns = L.lo;
+  B.out[20 * L.deal + 4 * L.strain + L.decl] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
+  L.decl++;
+  if (L.decl <= L.last_decl) {
+    L.prev_ns = ns < 1 ? 1 : ns;
+    L.stage = ST_ROOT;
+    return;
+  }
+  L.stage = ST_JOB;
+}
+
+// ENTER: enter the node at the current position; `st` is the lane's frame stack
+// (st.store(i, f) / st.load(i)).
+template <class Stack>
+DDX_FN void lane_enter(Lane& L, const Batch& B, Stack& st) {
+  Frame f;
+  u64 rel = 0;
+  const int e = enter(L.c, L.p, L.target, f, rel, L.exit_at);
+  if (e >= 0) {
+    L.v = e;
+    L.r = rel;
+    if (L.depth == 0) lane_search_done(L, B);
+    else L.stage = ST_RET;
+    return;
+  }
+  if (L.depth > 0) st.store(L.depth - 1, L.top);
+  L.top = f;
+  L.depth++;
+  L.stage = ST_NEXT;
+}
+
+// The top frame finished with (L.v, L.r).
+template <class Stack>
+DDX_FN void lane_node_done(Lane& L, const Batch& B, Stack& st) {
+  L.depth--;
+  if (L.depth == 0) {
+    lane_search_done(L, B);
+    return;
+  }
+  L.top = st.load(L.depth - 1);
+  L.stage = ST_RET;
+}
+
+// NEXT: play the next move of the top frame, or finish it.
+template <class Stack>
+DDX_FN void lane_next(Lane& L, const Batch& B, Stack& st) {
+  const int card = next_move(L.top);
+  if (card >= 0) {
+    play(L.p, L.top, card);
+    L.stage = ST_ENTER;
+    return;
+  }
+  L.v = L.top.result;
+  L.r = finish(L.c, L.p, L.top);
+  lane_node_done(L, B, st);
+}
+
+// RET: the child of the top frame returned (L.v, L.r): undo its move; a result that reaches
+// the mover's goal cuts off, else collect its relevant cards and try the next move
+// (reference dd_search child_done, src/dd.h:840-854).
+template <class Stack>
+DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
+  Frame& f = L.top;
+  const int card = move_at(f, f.i);
+  unplay(L.p, f, card, L.r);
+  if (L.v == ((f.seat & 1) == 0 ? 1 : 0)) {
+    f.result = L.v;
+    f.acc = L.r;
+    f.best = card;
+    L.r = finish(L.c, L.p, f);
+    lane_node_done(L, B, st);
+    return;
+  }
+  f.acc |= L.r;
+  f.i++;
+  L.stage = ST_NEXT;
+}
+
+// One stage of the machine. `take` returns the next job index (>= jobs: none left).
+template <class Stack, class Take>
+DDX_FN void lane_step(Lane& L, const Batch& B, Stack& st, Take take) {
+  switch (L.stage) {
+    case ST_JOB: {
+      long long job = take();
+      if (job >= B.jobs) {
+        L.stage = ST_EXIT;
+        return;
+      }
+      lane_load_job(L, B, job);
+      return;
+    }
+    case ST_ROOT:
+      lane_root(L, B);
+      return;
+    case ST_ENTER:
+      lane_enter(L, B, st);
+      return;
+    case ST_NEXT:
+      lane_next(L, B, st);
+      return;
+    case ST_RET:
+      lane_ret(L, B, st);
+      return;
+    default:
+      return;
+  }
+}
+
+// A frame stack in ordinary memory (CPU, tests).
+struct ArrayStack {
+  PackedFrame f[52];
+  DDX_M void store(int i, const Frame& x) { f[i] = pack_frame(x); }
+  DDX_M Frame load(int i) const { return unpack_frame(f[i]); }
+};
+
+}  // namespace ddx
