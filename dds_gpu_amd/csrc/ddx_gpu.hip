@@ -1,0 +1,474 @@
+// GPU side of the ddx C ABI: the wavefront solver kernel for gfx950 and its host driver.
+//
+// One thread = one solver lane with a private TT slice and frame stack in HBM; lanes take
+// (deal, strain) jobs from one atomic counter until none is left (persistent grid). The
+// machine is the one the CPU runs (include/ddx/machine.hpp), so results are bit-identical.
+//
+// Per-lane device memory (L lanes, B = 2^tt_log2 buckets):
+//   tags   L * B * 64 B    tag lines, lane-major (lane l's TT is one contiguous range)
+//   ents   L * B * 256 B   entries, lane-major
+//   stack  L * 52 * 32 B   packed frames, lane-major: a lane's neighbouring depths share
+//                          128-byte lines, so push/pop pairs hit the same line
+//   gen    L * 4 B         TT generation of each lane, kept across launches so tags of a
+//                          previous call never match (no per-call clear of the tag lines)
+//
+// Wavefront schedules (one wave64 = 64 lanes in lockstep; stages diverge per lane):
+//   PRIORITY  each round runs one stage for the lanes in it: RET > NEXT > ROOT > JOB > ENTER
+//             (the reference's order, gpu/gpu_main.cu:118-132)
+//   CHAIN     each lane runs its cheap stages (JOB/ROOT/NEXT/RET) back to back until it
+//             reaches ENTER; then the wave enters nodes for all lanes together
+//   SPLIT     as CHAIN, but follow nodes (2nd-4th hand, cheap) are entered in rounds of their
+//             own, so trick-start nodes (TT probe, quick/later tricks) gather and run with
+//             most of the wave active and their HBM probes issued together
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <new>
+
+#include "ddx.h"
+#include "ddx/machine.hpp"
+#include "ddx_internal.h"
+
+using namespace ddx;
+
+#define HIPCHECK(x)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) {                                                             \
+      ddx_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));     \
+      return DDX_E_HIP;                                                                 \
+    }                                                                                   \
+  } while (0)
+
+constexpr int kStackDepth = 52;
+
+struct GpuStack {
+  PackedFrame* base;
+  __device__ __forceinline__ void store(int i, const Frame& f) { base[i] = pack_frame(f); }
+  __device__ __forceinline__ Frame load(int i) const { return unpack_frame(base[i]); }
+};
+
+struct KernelArgs {
+  const u64* hands;
+  uint8_t* out;
+  const uint8_t* seeds;
+  long long jobs;
+  int jobs_per_deal;
+  int seed_mode;
+  u32* tags;
+  Entry* ents;
+  PackedFrame* stacks;
+  u32* lane_gen;
+  int tt_log2;
+  unsigned long long* counters;  // [0] next job, [1] nodes, [2] probes, [3] hits, [4] jobs done
+  volatile int* stop;            // host-mapped; nonzero = stop
+  unsigned long long* prof;      // rounds [0..7], lanes [8..15], exits [16..22], cycles [24..31]
+};
+
+template <int SCHED, bool PROF>
+__global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t buckets = (size_t)1 << a.tt_log2;
+  TTView tt;
+  tt.tags = a.tags + (size_t)lane * buckets * kWays;
+  tt.ents = a.ents + (size_t)lane * buckets * kWays;
+  tt.mask = (u32)(buckets - 1);
+  GpuStack st{a.stacks + (size_t)lane * kStackDepth};
+  Lane L;
+  lane_init(L, tt);
+  L.c.gen = a.lane_gen[lane];
+  const Batch B{a.hands, a.out, a.seeds, a.jobs, a.jobs_per_deal, a.seed_mode};
+  unsigned long long* next = a.counters;
+  auto take = [&]() -> long long { return (long long)atomicAdd(next, 1ull); };
+  // Profile (PROF only; wave-uniform counters kept by every lane, lane 0 reports):
+  // kinds 0..5 = stage rounds of the PRIORITY schedule, 6 = cheap chains, 7 = follow ENTER,
+  // 2 = (trick-start) ENTER in the other schedules.
+  unsigned long long p_rounds[8], p_lanes[8], p_cycles[8];
+  unsigned p_exit[EX_COUNT];
+  if (PROF) {
+    for (int i = 0; i < 8; i++) p_rounds[i] = p_lanes[i] = p_cycles[i] = 0;
+    for (int i = 0; i < EX_COUNT; i++) p_exit[i] = 0;
+  }
+  auto prof_round = [&](int kind, bool active, long long t0) {
+    if (PROF) {
+      unsigned long long m = __ballot(active);
+      p_rounds[kind]++;
+      p_lanes[kind] += __popcll(m);
+      p_cycles[kind] += (unsigned long long)(clock64() - t0);
+    }
+  };
+  unsigned rounds = 0;
+
+  // Cheap stages of one lane, back to back, until it must enter a node or has no work.
+  auto cheap_chain = [&]() {
+    while (L.stage != ST_ENTER && L.stage != ST_EXIT) {
+      if (L.stage == ST_JOB) {
+        const long long job = take();
+        if (job >= a.jobs) {
+          L.stage = ST_EXIT;
+          break;
+        }
+        lane_load_job(L, B, job);
+      } else if (L.stage == ST_ROOT) {
+        lane_root(L, B);
+      } else if (L.stage == ST_NEXT) {
+        lane_next(L, B, st);
+      } else {
+        lane_ret(L, B, st);
+      }
+    }
+  };
+
+  for (;;) {
+    if ((++rounds & 4095) == 0 && *a.stop) L.stage = ST_EXIT;
+    if (SCHED == DDX_SCHED_PRIORITY) {
+      if (__all(L.stage == ST_EXIT)) break;
+      int pick;
+      if (__any(L.stage == ST_RET)) pick = ST_RET;
+      else if (__any(L.stage == ST_NEXT)) pick = ST_NEXT;
+      else if (__any(L.stage == ST_ROOT)) pick = ST_ROOT;
+      else if (__any(L.stage == ST_JOB)) pick = ST_JOB;
+      else pick = ST_ENTER;
+      const bool run = L.stage == pick;
+      const long long t0 = PROF ? clock64() : 0;
+      if (run) {
+        lane_step(L, B, st, take);
+        if (PROF && pick == ST_ENTER) p_exit[L.exit_at]++;
+      }
+      prof_round(pick, run, t0);
+    } else {
+      long long t0 = PROF ? clock64() : 0;
+      const bool cheap = L.stage != ST_ENTER && L.stage != ST_EXIT;
+      cheap_chain();
+      prof_round(6, cheap, t0);
+      if (__all(L.stage == ST_EXIT)) break;
+      if (SCHED == DDX_SCHED_SPLIT && __any(L.stage == ST_ENTER && L.p.nplayed != 0)) {
+        // Follow nodes first; trick-start lanes wait and gather.
+        const bool run = L.stage == ST_ENTER && L.p.nplayed != 0;
+        t0 = PROF ? clock64() : 0;
+        if (run) {
+          lane_enter(L, B, st);
+          if (PROF) p_exit[L.exit_at]++;
+        }
+        prof_round(7, run, t0);
+        continue;
+      }
+      const bool run = L.stage == ST_ENTER;
+      t0 = PROF ? clock64() : 0;
+      if (run) {
+        lane_enter(L, B, st);
+        if (PROF) p_exit[L.exit_at]++;
+      }
+      prof_round(ST_ENTER, run, t0);
+    }
+  }
+  a.lane_gen[lane] = L.c.gen;
+  atomicAdd(a.counters + 1, (unsigned long long)L.c.nodes);
+  atomicAdd(a.counters + 2, (unsigned long long)L.c.probes);
+  atomicAdd(a.counters + 3, (unsigned long long)L.c.hits);
+  atomicAdd(a.counters + 4, (unsigned long long)L.jobs_done);
+  if (PROF) {
+    if ((threadIdx.x & 63) == 0) {
+      for (int i = 0; i < 8; i++) {
+        atomicAdd(a.prof + i, p_rounds[i]);
+        atomicAdd(a.prof + 8 + i, p_lanes[i]);
+        atomicAdd(a.prof + 24 + i, p_cycles[i]);
+      }
+    }
+    for (int i = 0; i < EX_COUNT; i++)
+      if (p_exit[i]) atomicAdd(a.prof + 16 + i, (unsigned long long)p_exit[i]);
+  }
+}
+
+// Fills n 16-byte words with a 32-bit pattern (graph-capturable; no hipMemset nodes).
+__global__ void fill_kernel(Q4* p, size_t n, u32 v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  Q4 q = {v, v, v, v};
+  for (; i < n; i += stride) p[i] = q;
+}
+__global__ void fill_bytes_kernel(uint8_t* p, size_t n, uint8_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host driver.
+struct ddx_gpu {
+  ddx_gpu_config cfg;
+  u32* tags = nullptr;
+  Entry* ents = nullptr;
+  PackedFrame* stacks = nullptr;
+  u32* lane_gen = nullptr;
+  unsigned long long* counters = nullptr;
+  unsigned long long* prof = nullptr;
+  int* h_stop = nullptr;  // host-mapped
+  int* d_stop = nullptr;
+  // staging for ddx_gpu_solve (host buffers in, host buffers out)
+  u64* d_hands = nullptr;
+  uint8_t* d_out = nullptr;
+  uint8_t* d_seeds = nullptr;
+  int64_t cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream = nullptr;  // stream of the last launch
+  std::chrono::steady_clock::time_point t0;
+  double last_ms = 0;
+  int64_t last_jobs = 0;
+  uint64_t bytes = 0;
+};
+
+extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
+  cfg->device = 0;
+  cfg->lanes = 65536;
+  cfg->tt_log2 = 10;
+  cfg->jobs_per_deal = 5;
+  cfg->schedule = DDX_SCHED_CHAIN;
+  cfg->seed_mode = DDX_SEED_PLAIN;
+  cfg->block = 256;
+  cfg->profile = 0;
+}
+
+static void free_all(ddx_gpu* h) {
+  if (h->tags) (void)hipFree(h->tags);
+  if (h->ents) (void)hipFree(h->ents);
+  if (h->stacks) (void)hipFree(h->stacks);
+  if (h->lane_gen) (void)hipFree(h->lane_gen);
+  if (h->counters) (void)hipFree(h->counters);
+  if (h->prof) (void)hipFree(h->prof);
+  if (h->h_stop) (void)hipHostFree(h->h_stop);
+  if (h->d_hands) (void)hipFree(h->d_hands);
+  if (h->d_out) (void)hipFree(h->d_out);
+  if (h->d_seeds) (void)hipFree(h->d_seeds);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+}
+
+static int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s) {
+  size_t n = bytes / 16;
+  if (n) {
+    int grid = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, (Q4*)p, n, v);
+    HIPCHECK(hipGetLastError());
+  }
+  return DDX_OK;
+}
+
+extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
+  if (!out) return DDX_E_ARG;
+  ddx_gpu_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else ddx_gpu_default_config(&cfg);
+  if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 64 ||
+      (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.schedule < 0 ||
+      cfg.schedule > 2 || cfg.seed_mode < 0 || cfg.seed_mode > 3 || cfg.block < 64 ||
+      cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block) {
+    ddx_set_error("ddx_gpu_open: bad config (lanes %lld must be a multiple of block %d and of 64, "
+                  "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20)",
+                  (long long)cfg.lanes, cfg.block, cfg.tt_log2);
+    return DDX_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg.device) {
+    ddx_set_error("ddx_gpu_open: no HIP device %d", cfg.device);
+    return DDX_E_NODEV;
+  }
+  HIPCHECK(hipSetDevice(cfg.device));
+  ddx_gpu* h = new (std::nothrow) ddx_gpu;
+  if (!h) return DDX_E_NOMEM;
+  h->cfg = cfg;
+  const size_t lanes = (size_t)cfg.lanes, buckets = (size_t)1 << cfg.tt_log2;
+  const size_t tag_bytes = lanes * buckets * 64, ent_bytes = lanes * buckets * 256;
+  const size_t stack_bytes = lanes * kStackDepth * sizeof(PackedFrame);
+  auto fail = [&](const char* what) {
+    ddx_set_error("ddx_gpu_open: cannot allocate %s", what);
+    free_all(h);
+    delete h;
+    return DDX_E_NOMEM;
+  };
+  if (hipMalloc(&h->tags, tag_bytes) != hipSuccess) return fail("TT tag lines");
+  if (hipMalloc(&h->ents, ent_bytes) != hipSuccess) return fail("TT entries");
+  if (hipMalloc(&h->stacks, stack_bytes) != hipSuccess) return fail("frame stacks");
+  if (hipMalloc(&h->lane_gen, lanes * 4 + 64) != hipSuccess) return fail("lane generations");
+  if (hipMalloc(&h->counters, 8 * 8) != hipSuccess) return fail("counters");
+  if (hipMalloc(&h->prof, 32 * 8) != hipSuccess) return fail("profile counters");
+  if (hipHostMalloc((void**)&h->h_stop, 64, hipHostMallocMapped) != hipSuccess) return fail("stop flag");
+  *h->h_stop = 0;
+  if (hipHostGetDevicePointer((void**)&h->d_stop, h->h_stop, 0) != hipSuccess) return fail("stop flag");
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return fail("events");
+  h->bytes = tag_bytes + ent_bytes + stack_bytes + lanes * 4;
+  int rc;
+  if ((rc = launch_fill(h->tags, tag_bytes, 0, nullptr)) != DDX_OK ||
+      (rc = launch_fill(h->lane_gen, (lanes * 4 + 15) / 16 * 16, 0, nullptr)) != DDX_OK ||
+      (rc = launch_fill(h->prof, 32 * 8, 0, nullptr)) != DDX_OK) {
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    ddx_set_error("ddx_gpu_open: clearing the TT failed");
+    free_all(h);
+    delete h;
+    return DDX_E_HIP;
+  }
+  *out = h;
+  return DDX_OK;
+}
+
+extern "C" void ddx_gpu_close(ddx_gpu* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->cfg.device);
+  (void)hipDeviceSynchronize();
+  free_all(h);
+  delete h;
+}
+
+extern "C" uint64_t ddx_gpu_device_bytes(const ddx_gpu* h) { return h ? h->bytes : 0; }
+extern "C" void ddx_gpu_request_stop(ddx_gpu* h) {
+  if (h && h->h_stop) __atomic_store_n(h->h_stop, 1, __ATOMIC_SEQ_CST);
+}
+extern "C" double ddx_gpu_last_kernel_ms(const ddx_gpu* h) { return h ? h->last_ms : 0; }
+
+template <int S, bool P>
+static void launch_solve(const KernelArgs& a, int grid, int block, hipStream_t s) {
+  hipLaunchKernelGGL((solve_kernel<S, P>), dim3(grid), dim3(block), 0, s, a);
+}
+
+extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t n,
+                                   const uint8_t* d_seeds, uint8_t* d_out, void* stream) {
+  if (!h || n < 0 || (n && (!d_hands || !d_out)) ||
+      ((h->cfg.seed_mode == DDX_SEED_STEP || h->cfg.seed_mode == DDX_SEED_CERT) && !d_seeds && n)) {
+    ddx_set_error("ddx_gpu_solve_async: bad arguments");
+    return DDX_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  hipStream_t s = (hipStream_t)stream;
+  h->stream = s;
+  __atomic_store_n(h->h_stop, 0, __ATOMIC_SEQ_CST);
+  const long long jobs = (long long)n * h->cfg.jobs_per_deal;
+  h->last_jobs = jobs;
+  int rc;
+  if ((rc = launch_fill(h->counters, 64, 0, s)) != DDX_OK) return rc;
+  if (n) {
+    int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, d_out, (size_t)(20 * n),
+                       (uint8_t)0xFF);
+    HIPCHECK(hipGetLastError());
+  }
+  KernelArgs a;
+  a.hands = d_hands;
+  a.out = d_out;
+  a.seeds = d_seeds;
+  a.jobs = jobs;
+  a.jobs_per_deal = h->cfg.jobs_per_deal;
+  a.seed_mode = h->cfg.seed_mode;
+  a.tags = h->tags;
+  a.ents = h->ents;
+  a.stacks = h->stacks;
+  a.lane_gen = h->lane_gen;
+  a.tt_log2 = h->cfg.tt_log2;
+  a.counters = h->counters;
+  a.stop = h->d_stop;
+  a.prof = h->prof;
+  const int block = h->cfg.block;
+  const int grid = (int)(h->cfg.lanes / block);
+  h->t0 = std::chrono::steady_clock::now();
+  HIPCHECK(hipEventRecord(h->ev0, s));
+  const bool P = h->cfg.profile != 0;
+  switch (h->cfg.schedule * 2 + (P ? 1 : 0)) {
+    case 0: launch_solve<DDX_SCHED_PRIORITY, false>(a, grid, block, s); break;
+    case 1: launch_solve<DDX_SCHED_PRIORITY, true>(a, grid, block, s); break;
+    case 2: launch_solve<DDX_SCHED_CHAIN, false>(a, grid, block, s); break;
+    case 3: launch_solve<DDX_SCHED_CHAIN, true>(a, grid, block, s); break;
+    case 4: launch_solve<DDX_SCHED_SPLIT, false>(a, grid, block, s); break;
+    default: launch_solve<DDX_SCHED_SPLIT, true>(a, grid, block, s); break;
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(h->ev1, s));
+  return DDX_OK;
+}
+
+extern "C" int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats) {
+  if (!h) return DDX_E_ARG;
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  HIPCHECK(hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  unsigned long long c[8];
+  HIPCHECK(hipMemcpy(c, h->counters, sizeof c, hipMemcpyDeviceToHost));
+  if (stats) {
+    stats->nodes = c[1];
+    stats->tt_probes = c[2];
+    stats->tt_hits = c[3];
+    stats->jobs_done = c[4];
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h->t0).count();
+  }
+  if ((int64_t)c[4] < h->last_jobs) {
+    ddx_set_error("stopped after %llu of %lld jobs", c[4], (long long)h->last_jobs);
+    return DDX_E_STOPPED;
+  }
+  return DDX_OK;
+}
+
+static int ensure_staging(ddx_gpu* h, int64_t n) {
+  if (n <= h->cap) return DDX_OK;
+  if (h->d_hands) (void)hipFree(h->d_hands);
+  if (h->d_out) (void)hipFree(h->d_out);
+  if (h->d_seeds) (void)hipFree(h->d_seeds);
+  h->d_hands = nullptr;
+  h->d_out = h->d_seeds = nullptr;
+  h->cap = 0;
+  HIPCHECK(hipMalloc(&h->d_hands, (size_t)n * 32));
+  HIPCHECK(hipMalloc(&h->d_out, (size_t)n * 20));
+  HIPCHECK(hipMalloc(&h->d_seeds, (size_t)n * 20));
+  h->cap = n;
+  return DDX_OK;
+}
+
+extern "C" int ddx_gpu_solve(ddx_gpu* h, const uint64_t* hands, int64_t n, const uint8_t* seeds,
+                             uint8_t* out, ddx_stats* stats) {
+  if (!h || n < 0 || (n && (!hands || !out))) {
+    ddx_set_error("ddx_gpu_solve: bad arguments");
+    return DDX_E_ARG;
+  }
+  for (int64_t i = 0; i < n; i++)
+    if (!ddx_check_deal(hands + 4 * i)) {
+      ddx_set_error("ddx_gpu_solve: deal %lld is not a valid deal", (long long)i);
+      return DDX_E_ARG;
+    }
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  int rc = ensure_staging(h, n > 0 ? n : 1);
+  if (rc != DDX_OK) return rc;
+  if (n) HIPCHECK(hipMemcpy(h->d_hands, hands, (size_t)n * 32, hipMemcpyHostToDevice));
+  const bool want_seeds = h->cfg.seed_mode == DDX_SEED_STEP || h->cfg.seed_mode == DDX_SEED_CERT;
+  if (want_seeds) {
+    if (!seeds) {
+      ddx_set_error("ddx_gpu_solve: this seed mode needs seeds");
+      return DDX_E_ARG;
+    }
+    if (n) HIPCHECK(hipMemcpy(h->d_seeds, seeds, (size_t)n * 20, hipMemcpyHostToDevice));
+  }
+  rc = ddx_gpu_solve_async(h, h->d_hands, n, want_seeds ? h->d_seeds : nullptr, h->d_out, nullptr);
+  if (rc != DDX_OK) return rc;
+  int wrc = ddx_gpu_wait(h, stats);
+  if (wrc != DDX_OK && wrc != DDX_E_STOPPED) return wrc;
+  if (n) HIPCHECK(hipMemcpy(out, h->d_out, (size_t)n * 20, hipMemcpyDeviceToHost));
+  return wrc;
+}
+
+// Per-stage profile of the last solves (config.profile = 1).
+extern "C" int ddx_gpu_profile(ddx_gpu* h, uint64_t* out32, int reset) {
+  if (!h || !out32) return DDX_E_ARG;
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  HIPCHECK(hipMemcpy(out32, h->prof, 32 * 8, hipMemcpyDeviceToHost));
+  if (reset) {
+    int rc = launch_fill(h->prof, 32 * 8, 0, nullptr);
+    if (rc != DDX_OK) return rc;
+    HIPCHECK(hipDeviceSynchronize());
+  }
+  return DDX_OK;
+}

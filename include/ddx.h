@@ -148,6 +148,12 @@ int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats);
 void ddx_gpu_request_stop(ddx_gpu* h);
 /* Duration in ms of the last solve kernel, from HIP events on its stream. */
 double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
+/* Per-stage counters of the last solves (config.profile = 1), per wave64: out32[0..7] rounds by
+ * kind (0 JOB, 1 ROOT, 2 ENTER -- trick-start ENTER under CHAIN/SPLIT --, 3 NEXT, 4 RET,
+ * 6 cheap-stage chains, 7 follow ENTER), out32[8..15] active lanes summed over those rounds,
+ * out32[16..22] where ENTER decided nodes (trivial, last trick, TT, quick tricks, later tricks,
+ * 2nd-hand quick tricks, moves), out32[24..31] shader-clock cycles by kind. reset != 0 clears. */
+int ddx_gpu_profile(ddx_gpu* h, uint64_t* out32, int reset);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,15 @@ DDX_FN int owner(const u64* h, int card) {
   return (int)(((h[1] >> card) & 1) | (((h[2] >> card) & 1) << 1) | (((h[3] >> card) & 1) * 3));
 }
 
+// Hand of a seat chosen at run time, by masks. Indexing the 4-hand array with a run-time
+// seat -- or a select between its elements, which the compiler turns back into a load from a
+// run-time address -- would force the hands (and the structs holding them) out of registers
+// into scratch memory on the GPU.
+DDX_FN u64 seat_hand(const u64* h, int seat) {
+  return (h[0] & (0ull - (u64)(seat == 0))) | (h[1] & (0ull - (u64)(seat == 1))) |
+         (h[2] & (0ull - (u64)(seat == 2))) | (h[3] & (0ull - (u64)(seat == 3)));
+}
+
 // Trick-winner order: same suit -> higher rank, else only a trump beats
 // (reference dd_beats, src/dd.h:537-541).
 DDX_FN bool beats(int card, int best, int trump) {

@@ -392,7 +392,7 @@ DDX_FN int quick_tricks(const u64* h, const Top& t, int trump, int hand, int cut
     if (side && cO > 0 && ruffed == 0 &&
         (qt == 0 || (wh != hand && wh != P && win_h(t, trump) != hand && win_h(t, trump) != P)) &&
         cP == 0 && tlen(t, P, trump) > 0) {
-      u64 T = suit_mask(trump), pt = h[P] & T;
+      u64 T = suit_mask(trump), pt = seat_hand(h, P) & T;
       if ((cR > 0 || tlen(t, R, trump) == 0) && (cL > 0 || tlen(t, L, trump) == 0)) {
         ruffed = 1;
         if (1 >= cutoff) return 1;
@@ -400,7 +400,8 @@ DDX_FN int quick_tricks(const u64* h, const Top& t, int trump, int hand, int cut
         continue;
       }
       // Some opponent can overruff: partner's top trump must beat theirs.
-      u64 over = cR == 0 && cL == 0 ? (h[L] | h[R]) & T : cL == 0 ? h[L] & T : h[R] & T;
+      const u64 hL = seat_hand(h, L), hR = seat_hand(h, R);
+      u64 over = cR == 0 && cL == 0 ? (hL | hR) & T : cL == 0 ? hL & T : hR & T;
       if (over < pt) {
         ruffed = 1;
         wr |= bit(msb(pt));
@@ -433,14 +434,15 @@ DDX_FN bool quick_tricks_2nd(const u64* h, const Top& t, int trump, int hand, in
                              u64& wr) {
   const int ss = lead >> 4, P = partner(hand), L = lho(hand);
   const u64 S = suit_mask(ss);
-  const u64 ranks = (h[hand] | h[P]) & S;
+  const u64 hO = seat_hand(h, hand), hP = seat_hand(h, P), hL = seat_hand(h, L);
+  const u64 ranks = (hO | hP) & S;
   wr = 0;
   const bool side = trump >= 0 && ss != trump;
   const u64 T = trump >= 0 ? suit_mask(trump) : 0;
-  const bool l_ruffs = !(h[L] & S) && (h[L] & T);
-  if (side && ((!(h[hand] & S) && (h[hand] & T)) || (!(h[P] & S) && (h[P] & T)))) {
+  const bool l_ruffs = !(hL & S) && (hL & T);
+  if (side && ((!(hO & S) && (hO & T)) || (!(hP & S) && (hP & T)))) {
     if (l_ruffs) return false;  // our side ruffs, but the 4th hand can overruff
-  } else if (ranks > (bit(lead) | (h[L] & S))) {
+  } else if (ranks > (bit(lead) | (hL & S))) {
     if (side && l_ruffs) return false;
     wr = bit(msb(ranks));
   } else {
@@ -449,16 +451,18 @@ DDX_FN bool quick_tricks_2nd(const u64* h, const Top& t, int trump, int hand, in
   int qt = 1;
   if (qt >= cutoff) return true;
   if (trump >= 0) return false;
-  const int hh = (h[hand] & S) > (h[P] & S) ? hand : P;
+  const bool own_hi = (hO & S) > (hP & S);
+  const int hh = own_hi ? hand : P;
+  const u64 hH = own_hi ? hO : hP, others = (h[0] | h[1] | h[2] | h[3]) & ~hH;
   if (win_h(t, ss) == hh && sec_h(t, ss) == hh) {
     wr |= bit(sec_c(t, ss));
     if (++qt >= cutoff) return true;
   }
 #pragma unroll
   for (int s = 0; s < 4; s++) {
-    if (s == ss || !(h[hh] & suit_mask(s))) continue;
-    if (!((h[lho(hh)] | h[rho(hh)] | h[partner(hh)]) & suit_mask(s))) {
-      qt += popc(h[hh] & suit_mask(s));
+    if (s == ss || !(hH & suit_mask(s))) continue;
+    if (!(others & suit_mask(s))) {
+      qt += popc(hH & suit_mask(s));
       if (qt >= cutoff) return true;
     } else if (win_h(t, s) == hh) {
       wr |= bit(win_c(t, s));
@@ -607,10 +611,11 @@ DDX_FN bool later_max(const Top& t, int trump, int hand, int ns_won, int left, i
 DDX_FN int lead_weight(const u64* h, const Top& t, int trump, int left, int lead, int card, bool tt) {
   const int s = card >> 4, P = partner(lead), L = lho(lead), R = rho(lead);
   const u64 S = suit_mask(s), all = h[0] | h[1] | h[2] | h[3];
+  const u64 hO = seat_hand(h, lead), hP = seat_hand(h, P), hL = seat_hand(h, L), hR = seat_hand(h, R);
   const int r_rank = popc(all & S & ~((1ull << card) - 1));  // 1 = the suit's top card
   const u64 below = all & S & ((1ull << card) - 1);
-  const bool seq = below && ((h[lead] >> msb(below)) & 1);
-  const u64 risP = h[P] & S, risL = h[L] & S, risR = h[R] & S;
+  const bool seq = below && ((hO >> msb(below)) & 1);
+  const u64 risP = hP & S, risL = hL & S, risR = hR & S;
   const int lenL = tlen(t, L, s), lenR = tlen(t, R, s), lenP = tlen(t, P, s);
   const int cntLH = (lenL == 0 ? left : lenL) << 2;
   const int cntRH = (lenR == 0 ? left : lenR) << 2;
@@ -621,7 +626,7 @@ DDX_FN int lead_weight(const u64* h, const Top& t, int trump, int left, int lead
     const u64 T = suit_mask(trump);
     const int tL = tlen(t, L, trump), tR = tlen(t, R, trump), tP = tlen(t, P, trump);
     int bonus = 0;
-    if (s != trump && ((!risL && (h[L] & T)) || (!risR && (h[R] & T)))) bonus = -12;
+    if (s != trump && ((!risL && (hL & T)) || (!risR && (hR & T)))) bonus = -12;
     if (s != trump && lenP == 0 && tP > 0 && lenR > 0) bonus += 17;
     if (wh == R || sh == R) {
       if (lenR != 1) bonus += -12;
@@ -638,7 +643,7 @@ DDX_FN int lead_weight(const u64* h, const Top& t, int trump, int left, int lead
         if (lenP != 0 || tP == 0) {
           win = opp_cannot_ruff;
         } else {
-          win = (lenL != 0 || (h[P] & T) > (h[L] & T)) && (lenR != 0 || (h[P] & T) > (h[R] & T));
+          win = (lenL != 0 || (hP & T) > (hL & T)) && (lenR != 0 || (hP & T) > (hR & T));
         }
       } else {
         win = true;
@@ -647,9 +652,9 @@ DDX_FN int lead_weight(const u64* h, const Top& t, int trump, int left, int lead
       win = s != trump ? opp_cannot_ruff : true;
     } else if (s != trump && lenP == 0 && tP != 0) {
       const bool l_ruff = lenL == 0 && tL != 0, r_ruff = lenR == 0 && tR != 0;
-      if (l_ruff && r_ruff) win = (h[P] & T) > ((h[L] | h[R]) & T);
-      else if (l_ruff) win = (h[P] & T) > (h[L] & T);
-      else if (r_ruff) win = (h[P] & T) > (h[R] & T);
+      if (l_ruff && r_ruff) win = (hP & T) > ((hL | hR) & T);
+      else if (l_ruff) win = (hP & T) > (hL & T);
+      else if (r_ruff) win = (hP & T) > (hR & T);
       else win = true;
     }
     int w;
@@ -778,37 +783,38 @@ DDX_FN int follow_weight(const Follow& f, const u64* h, int trump, int n, int se
   const u64 LS = suit_mask(ls);
   const int s = card >> 4, rank = rank_of(card);
   const int P = partner(lead), R = rho(lead);
+  const u64 hS = seat_hand(h, seat), hP = seat_hand(h, P), hR = seat_hand(h, R), hO = seat_hand(h, lead);
   const int r_rank = popc(f.all & suit_mask(s) & ~((1ull << card) - 1));
   const u64 below = f.allt & suit_mask(s) & ((1ull << card) - 1);
-  const bool seq = below && ((h[seat] >> msb(below)) & 1);
+  const bool seq = below && ((hS >> msb(below)) & 1);
   const int lead0 = f.lead0;
   const bool is_void = s != ls;
-  const int cnt = popc(h[seat] & suit_mask(s));
+  const int cnt = popc(hS & suit_mask(s));
   const bool sec_mine = follow_top2(f, s, 1) == seat;
   const bool win_mine = follow_top2(f, s, 0) == seat;
   const u64 T = trump >= 0 ? suit_mask(trump) : 0;
 
   if (n == 1) {  // second hand; P = third hand, R = fourth hand
-    const int max3rd = hi_rank(h[P] & LS), maxpd = hi_rank(h[R] & LS);
-    const int min3rd = lo_rank(h[P] & LS), minpd = lo_rank(h[R] & LS);
+    const int max3rd = hi_rank(hP & LS), maxpd = hi_rank(hR & LS);
+    const int min3rd = lo_rank(hP & LS), minpd = lo_rank(hR & LS);
     if (!is_void) {
       if (trump >= 0) {
         bool win = false;
         if (ls == trump) {
           win = (maxpd > lead0 && maxpd > max3rd) || (rank > lead0 && rank > max3rd);
         } else if (rank > lead0 && rank > max3rd) {
-          win = (max3rd != 0 || !(h[P] & T)) ||
-                (maxpd == 0 && (h[R] & T) && (h[R] & T) > (h[P] & T));
+          win = (max3rd != 0 || !(hP & T)) ||
+                (maxpd == 0 && (hR & T) && (hR & T) > (hP & T));
         } else if (maxpd > lead0 && maxpd > max3rd) {
-          win = max3rd != 0 || !(h[P] & T);
+          win = max3rd != 0 || !(hP & T);
         } else if (lead0 > maxpd && lead0 > max3rd && lead0 > rank) {
-          win = maxpd == 0 && (h[R] & T) && ((max3rd != 0 || !(h[P] & T)) || (h[R] & T) > (h[P] & T));
-        } else if (maxpd == 0 && (h[R] & T)) {
+          win = maxpd == 0 && (hR & T) && ((max3rd != 0 || !(hP & T)) || (hR & T) > (hP & T));
+        } else if (maxpd == 0 && (hR & T)) {
           win = true;
         }
         if (win) {
           if (min3rd > rank) return 40 + r_rank;
-          if (maxpd > lead0 && (h[lead] & LS) > (h[R] & LS)) return 41 + r_rank;
+          if (maxpd > lead0 && (hO & LS) > (hR & LS)) return 41 + r_rank;
           if (rank > lead0) {
             if (rank < maxpd) return 78 - rank;
             if (rank > max3rd) return 73 - rank;
@@ -831,22 +837,22 @@ DDX_FN int follow_weight(const Follow& f, const u64* h, int trump, int n, int se
     const int short2 = cnt == 2 && sec_mine ? -4 : 0;
     if (trump >= 0) {
       if (ls == trump) {
-        if ((h[R] & LS) > ((h[P] & LS) | lead_bit)) return -rank + add44;
+        if ((hR & LS) > ((hP & LS) | lead_bit)) return -rank + add44;
         return -rank + add36 + short2;
       }
       if (s != trump) {
         bool hi;
-        if (h[P] & LS) hi = (h[R] & LS) > ((h[P] & LS) | lead_bit) || (!(h[R] & LS) && (h[R] & T));
-        else hi = (!(h[R] & LS) && (h[R] & T) > (h[P] & T)) || (!(h[P] & T) && (h[R] & LS) > lead_bit);
+        if (hP & LS) hi = (hR & LS) > ((hP & LS) | lead_bit) || (!(hR & LS) && (hR & T));
+        else hi = (!(hR & LS) && (hR & T) > (hP & T)) || (!(hP & T) && (hR & LS) > lead_bit);
         return hi ? -rank + 60 + add44 : -rank - 2 + add36 + short2;
       }
-      if ((h[P] & LS) || (!(h[R] & LS) && (h[R] & T) && (h[R] & T) > (h[P] & T)))
+      if ((hP & LS) || (!(hR & LS) && (hR & T) && (hR & T) > (hP & T)))
         return 24 - rank + add44;
-      if (bit(card) > (h[P] & T)) return 24 - rank + add44;
+      if (bit(card) > (hP & T)) return 24 - rank + add44;
       return 15 - rank + add36 + short2;
     }
     int add;
-    if ((h[R] & LS) > ((h[P] & LS) | lead_bit)) {
+    if ((hR & LS) > ((hP & LS) | lead_bit)) {
       add = (cnt << 6) / 23;
       if (cnt == 2 && sec_mine) add += -2;
       else if (cnt == 1 && win_mine) add += -3;
@@ -859,11 +865,11 @@ DDX_FN int follow_weight(const Follow& f, const u64* h, int trump, int n, int se
   }
 
   if (n == 2) {  // third hand; R = fourth hand
-    const u64 c4 = h[R] & LS;
+    const u64 c4 = hR & LS;
     const int max4th = hi_rank(c4), min4th = lo_rank(c4);
     const int m1_rank = rank_of(f.move1), m1_suit = f.move1 >> 4;
     if (!is_void) {
-      const int max3rd = hi_rank(h[seat] & LS);
+      const int max3rd = hi_rank(hS & LS);
       if (trump >= 0) {
         if (ls == trump) {
           if (f.high1 == 0 && lead0 > max4th) return -rank;
@@ -889,16 +895,16 @@ DDX_FN int follow_weight(const Follow& f, const u64* h, int trump, int n, int se
     }
     if (trump >= 0) {
       if (ls == trump || s != trump) return -rank + (cnt << 6) / 40;
-      if (f.high1 == 0 && lead0 > max4th && (max4th != 0 || !(h[R] & T))) return -rank - 50;
+      if (f.high1 == 0 && lead0 > max4th && (max4th != 0 || !(hR & T))) return -rank - 50;
       if (m1_suit == trump && rank < m1_rank) return -32 + r_rank + (cnt << 6) / 40;
       const int add = (cnt << 6) / 50;
       if (f.high1 == 0) {
         if (max4th != 0) return (follow_top2(f, ls, 1) == lead ? 36 : 48) - rank + add;
-        if (bit(card) > (h[R] & T)) return 48 - rank + add;
+        if (bit(card) > (hR & T)) return 48 - rank + add;
         return -12 - rank + add;
       }
       if (max4th != 0) return 72 - rank + add;
-      if (bit(card) > (h[R] & T)) return 48 - rank + add;
+      if (bit(card) > (hR & T)) return 48 - rank + add;
       return 36 - rank + add;
     }
     int add = (cnt << 6) / 24;

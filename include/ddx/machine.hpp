@@ -50,6 +50,7 @@ struct Lane {
   long long deal;
   int strain, decl, last_decl, prev_ns, last_true;
   int exit_at;  // where the last enter() decided (profiling)
+  u32 jobs_done;
 };
 
 DDX_FN void lane_init(Lane& L, TTView tt) {
@@ -66,6 +67,7 @@ DDX_FN void lane_init(Lane& L, TTView tt) {
   L.last_true = 0;
   L.target = L.lo = L.hi = L.searches = 0;
   L.exit_at = 0;
+  L.jobs_done = 0;
 }
 
 DDX_FN void start_search(Lane& L) {
@@ -136,6 +138,7 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
     L.stage = ST_ROOT;
     return;
   }
+  L.jobs_done++;
   L.stage = ST_JOB;
 }
 

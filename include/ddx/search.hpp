@@ -48,12 +48,7 @@ DDX_FN int trick_winner(u32 trick, int trump) {
   }
   return w;
 }
-DDX_FN u64 hand_of(const Pos& p, int seat) {
-  u64 h = p.hand[0];
-#pragma unroll
-  for (int i = 1; i < 4; i++) h = seat == i ? p.hand[i] : h;
-  return h;
-}
+DDX_FN u64 hand_of(const Pos& p, int seat) { return seat_hand(p.hand, seat); }
 DDX_FN void toggle(Pos& p, int seat, u64 m) {
 #pragma unroll
   for (int i = 0; i < 4; i++) p.hand[i] ^= seat == i ? m : 0;

@@ -142,10 +142,10 @@ DDX_FN u32 match_mask(const TagLine& L, u32 tag) {
   for (int i = 0; i < kWays; i++) m |= (u32)((L.t[i] & ~0xF00u) == tag) << i;
   return m;
 }
-DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, register selects only
+DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, by masks (see seat_hand)
   u32 v = 0;
 #pragma unroll
-  for (int i = 0; i < kWays; i++) v = i == pos ? L.t[i] : v;
+  for (int i = 0; i < kWays; i++) v |= L.t[i] & (0u - (u32)(i == pos));
   return v;
 }
 

@@ -1,0 +1,45 @@
+"""The C ABI: libddx.so loads and exports every function include/ddx.h declares."""
+import ctypes
+import os
+import re
+import subprocess
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "ddx.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ddx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for must in ["ddx_cpu_solve_strain", "ddx_cpu_solve_strain_guess", "ddx_cpu_ns_tricks",
+                 "ddx_gpu_open", "ddx_gpu_solve", "ddx_gpu_request_stop", "ddx_last_error",
+                 "ddx_stream_open", "ddx_stream_push", "ddx_stream_poll", "ddx_stream_close"]:
+        assert must in names, must
+
+
+def test_library_exports_every_declared_symbol(ddx):
+    lib = ddx.lib()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", ddx.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r"\bT (ddx_\w+)", out))
+    assert set(declared()) <= exported
+
+
+def test_no_gpu_calls_without_a_device(ddx):
+    """Without a GPU the GPU entry points fail cleanly (no crash, an error message)."""
+    import torch
+    if torch.cuda.is_available():
+        return
+    cfg = ddx.GpuConfig()
+    ddx.lib().ddx_gpu_default_config(ctypes.byref(cfg))
+    h = ctypes.c_void_p()
+    rc = ddx.lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc != 0 and ddx.lib().ddx_last_error()
