@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: full-deal double-dummy tables per second on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--deals D] [--mode plain|learned|both]
+
+Workload (the plain wavefront-kernel configuration): full 52-card deals of the seeded stream
+2026 (uniform Fisher-Yates, 13 cards per hand), 65,536 solver lanes, TT_LOG2 10 per lane,
+exact 16-byte TT entries; 20 results per deal. A step is one run-to-completion pass of the
+solver over the whole batch (tail included): the hands are resident in HBM when the timed
+region starts and the 20-byte tables stay there. `value` counts whole tables per second.
+
+Checks and side measurements in the same JSON line:
+  bit_exact     the first 2,000 deals' tables vs the reference CPU solver's
+                (tests/golden/full_2000.npz); a mismatch exits 1
+  roofline      the solve kernel's algorithmic TT bytes per launch (64 B per probed tag line,
+                16 B per entry read, 144 B per store, 52 B of deal I/O per table) over its
+                HIP-event duration, against the 8 TB/s HBM peak
+  cpu_baseline  the reference's own CPU solver (oracle/_ref/dd_cpu, built from the reference
+                sources) on a bounded sample of the same deals, all host cores, TT_LOG2 18
+                (its default); it also verifies the GPU's tables of that sample
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "full-deal DD tables/sec on 1 MI355X, learned-seeded vs plain, bit-exact; p99 ms"
+STREAM_SEED = 2026
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--deals", type=int, default=100000)
+    ap.add_argument("--warmup-deals", type=int, default=2048)
+    ap.add_argument("--lanes", type=int, default=65536)
+    ap.add_argument("--tt-log2", type=int, default=10)
+    ap.add_argument("--jobs-per-deal", type=int, default=5)
+    ap.add_argument("--schedule", type=int, default=2)
+    ap.add_argument("--mode", default="plain", choices=["plain", "learned", "both"])
+    ap.add_argument("--cpu-sample", type=int, default=-1, help="deals for the CPU baseline (-1: auto)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(ddx, hands, tables, sample):
+    """Times the reference's dd_cpu on `sample` deals; returns the JSON object or None."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "dd_cpu")
+    if not os.path.exists(exe):
+        log("cpu_baseline: oracle/_ref/dd_cpu is not built; skipped")
+        return None
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "sample.npy")
+        ddx.pgx_write(path, hands[:sample], tables[:sample])
+        t0 = time.time()
+        r = subprocess.run([exe, path, "0", str(sample), str(cores), "18"], capture_output=True,
+                           text=True, timeout=600)
+        wall = time.time() - t0
+    line = r.stdout.strip().splitlines()[-1] if r.stdout.strip() else ""
+    log("cpu_baseline:", line)
+    try:
+        secs = float(line.split(":")[1].split("s,")[0])
+    except (IndexError, ValueError):
+        secs = wall
+    return {"value": round(sample / secs, 3), "unit": "tables/s", "cores": cores, "kind": "reference",
+            "sample": "first %d deals of the same stream, reference dd_cpu (cpu/cpu_main.cpp) built "
+                      "from the reference sources, %d threads, TT_LOG2 18 (its default); %.1f s; "
+                      "it checked the GPU's tables of the sample: %s" %
+                      (sample, cores, secs, "0 wrong" if r.returncode == 0 else "MISMATCH rc=%d" % r.returncode),
+            "verified_gpu_tables": r.returncode == 0}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; the solver shards deals across GPUs with no "
+              "exchange (see DESIGN.md), but only --gpus 1 is run here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from dds_gpu_amd import ddx
+
+    if not torch.cuda.is_available():
+        log("bench.py needs a GPU")
+        sys.exit(1)
+    n, W = a.deals, a.warmup_deals
+    log("generating %d deals" % (n + W))
+    hands = ddx.random_deals(STREAM_SEED, n, 13)
+    warm = ddx.random_deals(STREAM_SEED, W, 13, first=n)
+    dev = torch.device("cuda:0")
+    d_hands = torch.from_numpy(hands.view(np.int64)).to(dev)
+    d_warm = torch.from_numpy(warm.view(np.int64)).to(dev)
+    d_out = torch.empty((n, 20), dtype=torch.uint8, device=dev)
+    d_wout = torch.empty((W, 20), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    g = ddx.GpuSolver(lanes=a.lanes, tt_log2=a.tt_log2, jobs_per_deal=a.jobs_per_deal,
+                      schedule=a.schedule)
+    log("lanes %d, TT_LOG2 %d, %.1f GB of lane state" % (a.lanes, a.tt_log2, g.device_bytes / 1e9))
+    for i in range(a.warmup):
+        g.solve_async(d_warm.data_ptr(), W, d_wout.data_ptr(), stream=stream)
+        st = g.wait()
+        log("warmup step %d: %d deals, %.1f s" % (i, W, g.last_kernel_ms / 1e3))
+    torch.cuda.synchronize()
+    kernel_ms, stats = [], []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        g.solve_async(d_hands.data_ptr(), n, d_out.data_ptr(), stream=stream)
+        done_at = time.time()
+        while True:  # progress lines while the step runs
+            t = time.time()
+            if g.progress() >= n * a.jobs_per_deal:
+                break
+            if t - done_at > 30:
+                log("  step %d: %d of %d jobs" % (i, g.progress(), n * a.jobs_per_deal))
+                done_at = t
+            time.sleep(0.5)
+        st = g.wait()
+        kernel_ms.append(g.last_kernel_ms)
+        stats.append(st)
+        log("step %d: %.1f s kernel, %.3g nodes" % (i, g.last_kernel_ms / 1e3, st.nodes))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = d_out.cpu().numpy()
+    ms_per_step = wall * 1e3 / a.steps
+    value = n / (wall / a.steps)
+
+    # Exactness on the first 2,000 deals against the reference solver's tables.
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "full_2000.npz"))
+    m = min(n, len(gold["tables"]))
+    assert np.array_equal(gold["hands"][:m], hands[:m]), "deal stream differs from the fixture"
+    wrong = int((out[:m] != gold["tables"][:m]).sum())
+    unsolved = int((out == 255).any(axis=1).sum())
+
+    st = stats[-1]
+    kms = kernel_ms[-1]
+    alg_bytes = 64.0 * st.tt_probes + 16.0 * st.tt_entry_reads + 144.0 * st.tt_stores + 52.0 * n
+    achieved = alg_bytes / (kms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "solve_kernel_hbm_traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("deals") == n and tr.get("lanes") == a.lanes:
+            traffic = tr.get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "algorithmic_bytes": alg_bytes, "kernel_ms": round(kms, 1),
+                "note": "integer tree search; TT probes are dependent random 64 B reads, so the "
+                        "kernel is bound by instruction issue under divergence and memory latency, "
+                        "far below the HBM peak"}
+    cpu = None
+    if not a.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        sample = a.cpu_sample if a.cpu_sample > 0 else min(m, 8 * cores)
+        cpu = cpu_baseline(ddx, hands, out, sample)
+
+    res = {
+        "metric": METRIC, "value": round(value, 3), "unit": "tables/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic: seeded uniform full deals (stream %d)" % STREAM_SEED,
+        "config": {"workload": "plain wavefront kernel, %d full deals, %d lanes, TT_LOG2 %d, exact "
+                               "16-byte TT entries, run to completion" % (n, a.lanes, a.tt_log2),
+                   "deals": n, "lanes": a.lanes, "tt_log2": a.tt_log2,
+                   "jobs_per_deal": a.jobs_per_deal, "schedule": a.schedule, "seed_mode": "plain"},
+        "bit_exact": {"checked_answers": 20 * m, "wrong": wrong, "unsolved_tables": unsolved,
+                      "against": "reference CPU solver tables of the first %d deals" % m},
+        "nodes_per_table": round(st.nodes / n, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    g.close()
+    if wrong or unsolved or (cpu and not cpu["verified_gpu_tables"]):
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

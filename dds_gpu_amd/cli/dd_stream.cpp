@@ -1,0 +1,221 @@
+// dd_stream: the streaming solver on the MI355X.
+//   dd_stream                        deals from stdin, 32 bytes each (4 x little-endian uint64
+//                                    hands N,E,S,W); one 28-byte record per deal on stdout as
+//                                    soon as it is solved (any order): int64 0-based input
+//                                    index + 20 bytes tricks[strain C,D,H,S,NT][declarer N,E,S,W]
+//   dd_stream bench SECONDS [TT_LOG2] [CHECK_EVERY]
+//                                    random full deals; deals/s every 10 s; every CHECK_EVERY-th
+//                                    deal (default 50) re-solved on the CPU and compared
+// Same argv, records, stderr lines and exit codes as the reference's gpu/stream_main.cu
+// (0 = no mismatch, 1 = mismatch or GPU error, 2 = usage / TT_LOG2 < 9). Environment:
+//   THREADS      solver lanes (default 32768, rounded up to 256)
+//   PER_LEAD     1 (default here): one job per lead; 0: one per strain. With the 4,096-slot
+//                ring, per-strain jobs would leave most lanes without work
+//   RING         ring slots (default 4096)
+//   RATE         bench: Poisson arrivals at RATE deals/s (default 0 = as fast as slots free)
+//   SEED         bench: deal stream seed (default 2026)
+// Bench prints the latency from arrival to result: p50 and p99 in ms.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "ddx.h"
+
+using Clock = std::chrono::steady_clock;
+
+static double since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+int main(int argc, char** argv) {
+  const bool bench = argc > 1 && !strcmp(argv[1], "bench");
+  if (argc > 1 && !bench) {
+    fprintf(stderr, "usage: %s < deals.bin > results.bin | %s bench SECONDS [TT_LOG2] [CHECK_EVERY]\n",
+            argv[0], argv[0]);
+    return 2;
+  }
+  const double seconds = bench && argc > 2 ? atof(argv[2]) : 0;
+  const int tt_log2 = bench && argc > 3 ? atoi(argv[3]) : 10;
+  if (tt_log2 < 9) {
+    fprintf(stderr, "TT_LOG2 must be >= %d\n", 9);
+    return 2;
+  }
+  const long check_every = bench && argc > 4 ? atol(argv[4]) : 50;
+  ddx_gpu_config cfg;
+  ddx_gpu_default_config(&cfg);
+  long long threads = getenv("THREADS") ? atoll(getenv("THREADS")) : 32768;
+  cfg.lanes = (threads + 255) / 256 * 256;
+  cfg.tt_log2 = tt_log2;
+  cfg.jobs_per_deal = getenv("PER_LEAD") && !atoi(getenv("PER_LEAD")) ? 5 : 20;
+  const int ring = getenv("RING") ? atoi(getenv("RING")) : 4096;
+  const double rate = getenv("RATE") ? atof(getenv("RATE")) : 0;
+  const uint64_t seed = getenv("SEED") ? strtoull(getenv("SEED"), 0, 10) : 2026;
+  fprintf(stderr, "%lld threads, per-lane TT %.2f GB total, %d jobs per deal, ring %d deals\n",
+          (long long)cfg.lanes, 320.0 * cfg.lanes * (1 << tt_log2) / 1e9, cfg.jobs_per_deal, ring);
+  ddx_stream* s = nullptr;
+  if (ddx_stream_open(&cfg, ring, &s) != DDX_OK) {
+    fprintf(stderr, "%s\n", ddx_last_error());
+    return 1;
+  }
+  const auto t0 = Clock::now();
+  std::mutex mu;
+  std::vector<double> arrival;  // by deal id
+  std::vector<uint64_t> kept;   // bench: hands of the deals to check, by id / check_every
+  std::atomic<long long> produced{0};
+  std::atomic<bool> producing{true}, input_done{false};
+
+  std::thread producer([&] {
+    std::mt19937_64 rng(seed ^ 0x5DEECE66Dull);
+    std::exponential_distribution<double> gap(rate > 0 ? rate : 1.0);
+    double next_arrival = 0;
+    uint64_t h[4];
+    for (long long p = 0; producing; p++) {
+      if (bench) {
+        ddx_random_deals(seed, p, 1, 13, h);
+        if (rate > 0) {
+          next_arrival += gap(rng);
+          double wait = next_arrival - since(t0);
+          if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        }
+      } else if (fread(h, 8, 4, stdin) != 4) {
+        break;
+      }
+      const double arr = bench && rate > 0 ? next_arrival : since(t0);
+      {
+        std::lock_guard<std::mutex> g(mu);
+        arrival.push_back(arr);
+        if (bench && p % check_every == 0) kept.insert(kept.end(), h, h + 4);
+      }
+      int rc;
+      while ((rc = ddx_stream_push(s, h, nullptr, p)) == DDX_E_FULL) {
+        if (!producing) break;
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+      }
+      if (rc != DDX_OK) break;
+      produced = p + 1;
+    }
+    input_done = true;
+    if (!bench) ddx_stream_finish(s);
+  });
+
+  // Checker threads (bench): re-solve sampled deals on the CPU.
+  std::vector<std::vector<uint8_t>> to_check_tab;
+  std::vector<long long> to_check_id;
+  std::atomic<long> checked{0}, wrong{0};
+  std::atomic<bool> checking{true};
+  std::mutex qm;
+  std::vector<std::thread> checkers;
+  if (bench) {
+    int nc = std::max(1, (int)std::thread::hardware_concurrency() - 2);
+    for (int t = 0; t < nc; t++)
+      checkers.emplace_back([&] {
+        ddx_cpu* c = nullptr;
+        if (ddx_cpu_create(12, &c) != DDX_OK) return;
+        for (;;) {
+          long long id = -1;
+          std::vector<uint8_t> got;
+          {
+            std::lock_guard<std::mutex> g(qm);
+            if (!to_check_id.empty()) {
+              id = to_check_id.back();
+              got = to_check_tab.back();
+              to_check_id.pop_back();
+              to_check_tab.pop_back();
+            }
+          }
+          if (id < 0) {
+            if (!checking) break;
+            std::this_thread::sleep_for(std::chrono::milliseconds(20));
+            continue;
+          }
+          uint64_t h[4];
+          {
+            std::lock_guard<std::mutex> g(mu);
+            memcpy(h, &kept[4 * (id / check_every)], 32);
+          }
+          bool bad = false;
+          for (int st = 0; st < 5 && !bad; st++) {
+            int o4[4];
+            ddx_cpu_solve_strain(c, h, st, o4);
+            for (int d = 0; d < 4; d++) bad = bad || o4[d] != got[st * 4 + d];
+          }
+          checked++;
+          if (bad) wrong++;
+        }
+        ddx_cpu_destroy(c);
+      });
+  }
+
+  std::vector<double> lat;
+  std::vector<int64_t> ids(4096);
+  std::vector<uint8_t> tabs(4096 * 20);
+  long long consumed = 0, last_consumed = 0;
+  double last = 0;
+  int rc = 0;
+  for (;;) {
+    int got = 0;
+    if (ddx_stream_poll(s, 4096, ids.data(), tabs.data(), &got) != DDX_OK) {
+      rc = 1;
+      break;
+    }
+    const double t = since(t0);
+    for (int i = 0; i < got; i++) {
+      const long long id = ids[i];
+      double arr;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        arr = arrival[id];
+      }
+      lat.push_back(t - arr);
+      if (!bench) {
+        unsigned char rec[28];
+        memcpy(rec, &id, 8);
+        memcpy(rec + 8, &tabs[20 * i], 20);
+        fwrite(rec, 1, 28, stdout);
+      } else if (id % check_every == 0) {
+        std::lock_guard<std::mutex> g(qm);
+        to_check_id.push_back(id);
+        to_check_tab.emplace_back(tabs.begin() + 20 * i, tabs.begin() + 20 * i + 20);
+      }
+    }
+    consumed += got;
+    if (t - last >= 10) {
+      fprintf(stderr, "  %.0fs: %lld deals solved, last 10 s: %.1f deals/s, checked %ld wrong %ld\n", t,
+              consumed, (consumed - last_consumed) / (t - last), checked.load(), wrong.load());
+      last = t;
+      last_consumed = consumed;
+    }
+    if (bench && t > seconds) break;
+    if (!bench && input_done && consumed >= produced) break;
+    if (!got) std::this_thread::sleep_for(std::chrono::microseconds(200));
+  }
+  producing = false;
+  const double el = since(t0);
+  producer.join();
+  fflush(stdout);
+  ddx_stats st;
+  memset(&st, 0, sizeof st);
+  if (ddx_stream_close(s, &st) != DDX_OK) {
+    fprintf(stderr, "%s\n", ddx_last_error());
+    rc = 1;
+  }
+  checking = false;
+  for (auto& t : checkers) t.join();
+  std::sort(lat.begin(), lat.end());
+  auto pct = [&](double q) { return lat.empty() ? 0.0 : 1000 * lat[(size_t)(q * (lat.size() - 1))]; };
+  fprintf(stderr, "%lld deals solved in %.1f s (%.1f deals/s), CPU check: %ld wrong of %ld\n", consumed,
+          el, consumed / el, wrong.load(), checked.load());
+  fprintf(stderr, "latency ms: p50 %.0f p99 %.0f (arrival to result, %zu deals)\n", pct(0.5), pct(0.99),
+          lat.size());
+  return rc ? rc : wrong.load() != 0;
+}

@@ -27,28 +27,9 @@
 #include <chrono>
 #include <new>
 
-#include "ddx.h"
-#include "ddx/machine.hpp"
-#include "ddx_internal.h"
+#include "ddx_lanes.hpp"
 
 using namespace ddx;
-
-#define HIPCHECK(x)                                                                     \
-  do {                                                                                  \
-    hipError_t e_ = (x);                                                                \
-    if (e_ != hipSuccess) {                                                             \
-      ddx_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));     \
-      return DDX_E_HIP;                                                                 \
-    }                                                                                   \
-  } while (0)
-
-constexpr int kStackDepth = 52;
-
-struct GpuStack {
-  PackedFrame* base;
-  __device__ __forceinline__ void store(int i, const Frame& f) { base[i] = pack_frame(f); }
-  __device__ __forceinline__ Frame load(int i) const { return unpack_frame(base[i]); }
-};
 
 struct KernelArgs {
   const u64* hands;
@@ -57,12 +38,10 @@ struct KernelArgs {
   long long jobs;
   int jobs_per_deal;
   int seed_mode;
-  u32* tags;
-  Entry* ents;
-  PackedFrame* stacks;
-  u32* lane_gen;
-  int tt_log2;
-  unsigned long long* counters;  // [0] next job, [1] nodes, [2] probes, [3] hits, [4] jobs done
+  LaneView lanes;
+  unsigned long long* counters;  // [0] next job, [1] nodes, [2] probes, [3] hits, [4] jobs done,
+                                 // [5] progress (jobs done, updated while running),
+                                 // [6] TT entries read, [7] TT stores
   volatile int* stop;            // host-mapped; nonzero = stop
   unsigned long long* prof;      // rounds [0..7], lanes [8..15], exits [16..22], cycles [24..31]
 };
@@ -70,15 +49,9 @@ struct KernelArgs {
 template <int SCHED, bool PROF>
 __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t buckets = (size_t)1 << a.tt_log2;
-  TTView tt;
-  tt.tags = a.tags + (size_t)lane * buckets * kWays;
-  tt.ents = a.ents + (size_t)lane * buckets * kWays;
-  tt.mask = (u32)(buckets - 1);
-  GpuStack st{a.stacks + (size_t)lane * kStackDepth};
   Lane L;
-  lane_init(L, tt);
-  L.c.gen = a.lane_gen[lane];
+  GpuStack st;
+  lane_attach(a.lanes, lane, L, st);
   const Batch B{a.hands, a.out, a.seeds, a.jobs, a.jobs_per_deal, a.seed_mode};
   unsigned long long* next = a.counters;
   auto take = [&]() -> long long { return (long long)atomicAdd(next, 1ull); };
@@ -100,6 +73,7 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
     }
   };
   unsigned rounds = 0;
+  u32 reported = 0;  // finished jobs already added to the progress counter
 
   // Cheap stages of one lane, back to back, until it must enter a node or has no work.
   auto cheap_chain = [&]() {
@@ -123,6 +97,10 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
 
   for (;;) {
     if ((++rounds & 4095) == 0 && *a.stop) L.stage = ST_EXIT;
+    if (L.jobs_done != reported) {  // progress for the host (one atomic per finished job)
+      atomicAdd(a.counters + 5, (unsigned long long)(L.jobs_done - reported));
+      reported = L.jobs_done;
+    }
     if (SCHED == DDX_SCHED_PRIORITY) {
       if (__all(L.stage == ST_EXIT)) break;
       int pick;
@@ -164,11 +142,13 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
       prof_round(ST_ENTER, run, t0);
     }
   }
-  a.lane_gen[lane] = L.c.gen;
+  a.lanes.gen[lane] = L.c.gen;
   atomicAdd(a.counters + 1, (unsigned long long)L.c.nodes);
   atomicAdd(a.counters + 2, (unsigned long long)L.c.probes);
   atomicAdd(a.counters + 3, (unsigned long long)L.c.hits);
   atomicAdd(a.counters + 4, (unsigned long long)L.jobs_done);
+  atomicAdd(a.counters + 6, (unsigned long long)L.c.ent_reads);
+  atomicAdd(a.counters + 7, (unsigned long long)L.c.stores);
   if (PROF) {
     if ((threadIdx.x & 63) == 0) {
       for (int i = 0; i < 8; i++) {
@@ -182,7 +162,8 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
   }
 }
 
-// Fills n 16-byte words with a 32-bit pattern (graph-capturable; no hipMemset nodes).
+namespace ddx {
+
 __global__ void fill_kernel(Q4* p, size_t n, u32 v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -195,14 +176,63 @@ __global__ void fill_bytes_kernel(uint8_t* p, size_t n, uint8_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s) {
+  size_t n = bytes / 16;
+  if (n) {
+    int grid = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, (Q4*)p, n, v);
+    HIPCHECK(hipGetLastError());
+  }
+  return DDX_OK;
+}
+
+int lanes_alloc(LaneStore& s, long long lanes, int tt_log2) {
+  const size_t L = (size_t)lanes, buckets = (size_t)1 << tt_log2;
+  const size_t tag_bytes = L * buckets * 64, ent_bytes = L * buckets * 256;
+  const size_t stack_bytes = L * kStackDepth * sizeof(PackedFrame), gen_bytes = (L * 4 + 15) / 16 * 16;
+  s.lanes = lanes;
+  s.tt_log2 = tt_log2;
+  if (hipMalloc(&s.tags, tag_bytes) != hipSuccess || hipMalloc(&s.ents, ent_bytes) != hipSuccess ||
+      hipMalloc(&s.stacks, stack_bytes) != hipSuccess || hipMalloc(&s.gen, gen_bytes) != hipSuccess) {
+    lanes_free(s);
+    ddx_set_error("cannot allocate %.2f GB of lane state for %lld lanes at TT_LOG2 %d",
+                  (tag_bytes + ent_bytes + stack_bytes) / 1e9, lanes, tt_log2);
+    return DDX_E_NOMEM;
+  }
+  s.bytes = tag_bytes + ent_bytes + stack_bytes + gen_bytes;
+  int rc;
+  if ((rc = launch_fill(s.tags, tag_bytes, 0, nullptr)) != DDX_OK ||
+      (rc = launch_fill(s.gen, gen_bytes, 0, nullptr)) != DDX_OK) {
+    lanes_free(s);
+    return rc;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    lanes_free(s);
+    ddx_set_error("clearing the lanes' TT failed");
+    return DDX_E_HIP;
+  }
+  return DDX_OK;
+}
+
+void lanes_free(LaneStore& s) {
+  if (s.tags) (void)hipFree(s.tags);
+  if (s.ents) (void)hipFree(s.ents);
+  if (s.stacks) (void)hipFree(s.stacks);
+  if (s.gen) (void)hipFree(s.gen);
+  s.tags = nullptr;
+  s.ents = nullptr;
+  s.stacks = nullptr;
+  s.gen = nullptr;
+  s.bytes = 0;
+}
+
+}  // namespace ddx
+
 // ---------------------------------------------------------------------------------------
 // Host driver.
 struct ddx_gpu {
   ddx_gpu_config cfg;
-  u32* tags = nullptr;
-  Entry* ents = nullptr;
-  PackedFrame* stacks = nullptr;
-  u32* lane_gen = nullptr;
+  LaneStore lanes;
   unsigned long long* counters = nullptr;
   unsigned long long* prof = nullptr;
   int* h_stop = nullptr;  // host-mapped
@@ -214,6 +244,8 @@ struct ddx_gpu {
   int64_t cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipStream_t stream = nullptr;  // stream of the last launch
+  hipStream_t own = nullptr;     // non-blocking stream for calls that pass none
+  hipStream_t side = nullptr;    // non-blocking stream for progress reads
   std::chrono::steady_clock::time_point t0;
   double last_ms = 0;
   int64_t last_jobs = 0;
@@ -232,10 +264,7 @@ extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
 }
 
 static void free_all(ddx_gpu* h) {
-  if (h->tags) (void)hipFree(h->tags);
-  if (h->ents) (void)hipFree(h->ents);
-  if (h->stacks) (void)hipFree(h->stacks);
-  if (h->lane_gen) (void)hipFree(h->lane_gen);
+  lanes_free(h->lanes);
   if (h->counters) (void)hipFree(h->counters);
   if (h->prof) (void)hipFree(h->prof);
   if (h->h_stop) (void)hipHostFree(h->h_stop);
@@ -243,18 +272,11 @@ static void free_all(ddx_gpu* h) {
   if (h->d_out) (void)hipFree(h->d_out);
   if (h->d_seeds) (void)hipFree(h->d_seeds);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->own) (void)hipStreamDestroy(h->own);
+  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
 }
 
-static int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s) {
-  size_t n = bytes / 16;
-  if (n) {
-    int grid = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-    hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, (Q4*)p, n, v);
-    HIPCHECK(hipGetLastError());
-  }
-  return DDX_OK;
-}
 
 extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   if (!out) return DDX_E_ARG;
@@ -279,19 +301,17 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   ddx_gpu* h = new (std::nothrow) ddx_gpu;
   if (!h) return DDX_E_NOMEM;
   h->cfg = cfg;
-  const size_t lanes = (size_t)cfg.lanes, buckets = (size_t)1 << cfg.tt_log2;
-  const size_t tag_bytes = lanes * buckets * 64, ent_bytes = lanes * buckets * 256;
-  const size_t stack_bytes = lanes * kStackDepth * sizeof(PackedFrame);
   auto fail = [&](const char* what) {
     ddx_set_error("ddx_gpu_open: cannot allocate %s", what);
     free_all(h);
     delete h;
     return DDX_E_NOMEM;
   };
-  if (hipMalloc(&h->tags, tag_bytes) != hipSuccess) return fail("TT tag lines");
-  if (hipMalloc(&h->ents, ent_bytes) != hipSuccess) return fail("TT entries");
-  if (hipMalloc(&h->stacks, stack_bytes) != hipSuccess) return fail("frame stacks");
-  if (hipMalloc(&h->lane_gen, lanes * 4 + 64) != hipSuccess) return fail("lane generations");
+  int rc = lanes_alloc(h->lanes, cfg.lanes, cfg.tt_log2);
+  if (rc != DDX_OK) {
+    delete h;
+    return rc;
+  }
   if (hipMalloc(&h->counters, 8 * 8) != hipSuccess) return fail("counters");
   if (hipMalloc(&h->prof, 32 * 8) != hipSuccess) return fail("profile counters");
   if (hipHostMalloc((void**)&h->h_stop, 64, hipHostMallocMapped) != hipSuccess) return fail("stop flag");
@@ -299,20 +319,14 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   if (hipHostGetDevicePointer((void**)&h->d_stop, h->h_stop, 0) != hipSuccess) return fail("stop flag");
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return fail("events");
-  h->bytes = tag_bytes + ent_bytes + stack_bytes + lanes * 4;
-  int rc;
-  if ((rc = launch_fill(h->tags, tag_bytes, 0, nullptr)) != DDX_OK ||
-      (rc = launch_fill(h->lane_gen, (lanes * 4 + 15) / 16 * 16, 0, nullptr)) != DDX_OK ||
-      (rc = launch_fill(h->prof, 32 * 8, 0, nullptr)) != DDX_OK) {
+  if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess)
+    return fail("streams");
+  h->bytes = h->lanes.bytes;
+  if ((rc = launch_fill(h->prof, 32 * 8, 0, nullptr)) != DDX_OK || hipDeviceSynchronize() != hipSuccess) {
     free_all(h);
     delete h;
-    return rc;
-  }
-  if (hipDeviceSynchronize() != hipSuccess) {
-    ddx_set_error("ddx_gpu_open: clearing the TT failed");
-    free_all(h);
-    delete h;
-    return DDX_E_HIP;
+    return rc != DDX_OK ? rc : DDX_E_HIP;
   }
   *out = h;
   return DDX_OK;
@@ -345,7 +359,7 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
     return DDX_E_ARG;
   }
   HIPCHECK(hipSetDevice(h->cfg.device));
-  hipStream_t s = (hipStream_t)stream;
+  hipStream_t s = stream ? (hipStream_t)stream : h->own;
   h->stream = s;
   __atomic_store_n(h->h_stop, 0, __ATOMIC_SEQ_CST);
   const long long jobs = (long long)n * h->cfg.jobs_per_deal;
@@ -365,11 +379,7 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
   a.jobs = jobs;
   a.jobs_per_deal = h->cfg.jobs_per_deal;
   a.seed_mode = h->cfg.seed_mode;
-  a.tags = h->tags;
-  a.ents = h->ents;
-  a.stacks = h->stacks;
-  a.lane_gen = h->lane_gen;
-  a.tt_log2 = h->cfg.tt_log2;
+  a.lanes = lanes_view(h->lanes);
   a.counters = h->counters;
   a.stop = h->d_stop;
   a.prof = h->prof;
@@ -405,6 +415,8 @@ extern "C" int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats) {
     stats->tt_probes = c[2];
     stats->tt_hits = c[3];
     stats->jobs_done = c[4];
+    stats->tt_entry_reads = c[6];
+    stats->tt_stores = c[7];
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h->t0).count();
   }
   if ((int64_t)c[4] < h->last_jobs) {
@@ -452,12 +464,23 @@ extern "C" int ddx_gpu_solve(ddx_gpu* h, const uint64_t* hands, int64_t n, const
     }
     if (n) HIPCHECK(hipMemcpy(h->d_seeds, seeds, (size_t)n * 20, hipMemcpyHostToDevice));
   }
-  rc = ddx_gpu_solve_async(h, h->d_hands, n, want_seeds ? h->d_seeds : nullptr, h->d_out, nullptr);
+  rc = ddx_gpu_solve_async(h, h->d_hands, n, want_seeds ? h->d_seeds : nullptr, h->d_out, h->own);
   if (rc != DDX_OK) return rc;
   int wrc = ddx_gpu_wait(h, stats);
   if (wrc != DDX_OK && wrc != DDX_E_STOPPED) return wrc;
   if (n) HIPCHECK(hipMemcpy(out, h->d_out, (size_t)n * 20, hipMemcpyDeviceToHost));
   return wrc;
+}
+
+// Jobs finished so far by the running (or last) solve; safe to call while it runs.
+extern "C" int ddx_gpu_progress(ddx_gpu* h, uint64_t* jobs_done) {
+  if (!h || !jobs_done) return DDX_E_ARG;
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  unsigned long long v = 0;
+  HIPCHECK(hipMemcpyAsync(&v, h->counters + 5, 8, hipMemcpyDeviceToHost, h->side));
+  HIPCHECK(hipStreamSynchronize(h->side));
+  *jobs_done = v;
+  return DDX_OK;
 }
 
 // Per-stage profile of the last solves (config.profile = 1).

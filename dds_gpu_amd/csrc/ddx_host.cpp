@@ -243,7 +243,7 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
   const long long jobs = (long long)n * jobs_per_deal;
   Batch B{hands, out, seeds, jobs, jobs_per_deal, seed_mode};
   std::atomic<long long> next{0};
-  std::atomic<uint64_t> nodes{0}, probes{0}, hits{0};
+  std::atomic<uint64_t> nodes{0}, probes{0}, hits{0}, ent_reads{0}, stores{0};
   std::atomic<int> failed{0};
   auto t0 = std::chrono::steady_clock::now();
   auto worker = [&](bool record) {
@@ -270,6 +270,8 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
     nodes += L.c.nodes;
     probes += L.c.probes;
     hits += L.c.hits;
+    ent_reads += L.c.ent_reads;
+    stores += L.c.stores;
   };
   if (threads == 1) {
     worker(job_nodes != nullptr);
@@ -287,6 +289,8 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
     stats->tt_probes = probes;
     stats->tt_hits = hits;
     stats->jobs_done = (uint64_t)jobs;
+    stats->tt_entry_reads = ent_reads;
+    stats->tt_stores = stores;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return DDX_OK;

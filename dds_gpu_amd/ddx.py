@@ -21,7 +21,8 @@ _lib = None
 class Stats(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64),
                 ("tt_hits", ctypes.c_uint64), ("jobs_done", ctypes.c_uint64),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("tt_entry_reads", ctypes.c_uint64),
+                ("tt_stores", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +75,14 @@ def lib():
         L.ddx_gpu_last_kernel_ms.argtypes = [P]
         L.ddx_gpu_last_kernel_ms.restype = ctypes.c_double
         L.ddx_gpu_profile.argtypes = [P, P, I]
+        L.ddx_gpu_progress.argtypes = [P, P]
+        L.ddx_stream_open.argtypes = [P, I, P]
+        L.ddx_stream_push.argtypes = [P, P, P, I64]
+        L.ddx_stream_poll.argtypes = [P, I, P, P, P]
+        L.ddx_stream_in_flight.argtypes = [P]
+        L.ddx_stream_in_flight.restype = I64
+        L.ddx_stream_finish.argtypes = [P]
+        L.ddx_stream_close.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -212,6 +221,11 @@ class GpuSolver:
     def device_bytes(self):
         return lib().ddx_gpu_device_bytes(self.h)
 
+    def progress(self):
+        v = ctypes.c_uint64()
+        _check(lib().ddx_gpu_progress(self.h, ctypes.byref(v)))
+        return v.value
+
     def profile(self, reset=True):
         v = np.zeros(32, np.uint64)
         _check(lib().ddx_gpu_profile(self.h, _ptr(v), int(reset)))
@@ -241,3 +255,60 @@ def ns_tricks_of(tables):
     ns = t.copy()
     ns[:, :, 1::2] = 13 - t[:, :, 1::2]
     return ns.reshape(-1, 20)
+
+
+E_FULL = 1
+
+
+class Stream:
+    """The persistent streaming solver (ddx_stream_*): push deals, poll finished tables."""
+
+    def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=20, seed_mode=SEED_PLAIN,
+                 ring_slots=4096, device=0):
+        cfg = GpuConfig()
+        lib().ddx_gpu_default_config(ctypes.byref(cfg))
+        cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
+        cfg.jobs_per_deal, cfg.seed_mode = jobs_per_deal, seed_mode
+        self.h = ctypes.c_void_p()
+        _check(lib().ddx_stream_open(ctypes.byref(cfg), ring_slots, ctypes.byref(self.h)))
+        self._ids = np.zeros(4096, np.int64)
+        self._tab = np.zeros((4096, 20), np.uint8)
+
+    def push(self, hands4, deal_id, seeds20=None):
+        """True if queued, False if the ring is full."""
+        h = np.ascontiguousarray(hands4, np.uint64)
+        s = None if seeds20 is None else np.ascontiguousarray(seeds20, np.uint8)
+        rc = _check(lib().ddx_stream_push(self.h, _ptr(h), _ptr(s), deal_id), ok=(0, E_FULL))
+        return rc == 0
+
+    def poll(self):
+        got = ctypes.c_int()
+        _check(lib().ddx_stream_poll(self.h, len(self._ids), _ptr(self._ids), _ptr(self._tab),
+                                     ctypes.byref(got)))
+        n = got.value
+        return self._ids[:n].copy(), self._tab[:n].copy()
+
+    def in_flight(self):
+        return lib().ddx_stream_in_flight(self.h)
+
+    def finish(self):
+        _check(lib().ddx_stream_finish(self.h))
+
+    def close(self):
+        st = Stats()
+        if self.h:
+            h, self.h = self.h, ctypes.c_void_p()
+            _check(lib().ddx_stream_close(h, ctypes.byref(st)))
+        return st
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,7 +27,8 @@ enum {
   DDX_E_HIP = -3,     /* HIP runtime error */
   DDX_E_NODEV = -4,   /* no GPU, or the GPU code is missing */
   DDX_E_STOPPED = -5, /* ddx_gpu_request_stop() ended the run; unsolved results are 0xFF */
-  DDX_E_IO = -6       /* file format or I/O error */
+  DDX_E_IO = -6,      /* file format or I/O error */
+  DDX_E_FULL = 1      /* not an error: the stream's ring is full, poll and push again */
 };
 
 /* Seed policies for the trick-count search of each lead. */
@@ -45,6 +46,8 @@ typedef struct ddx_stats {
   uint64_t tt_hits;      /* probes that decided the node (reference DDCtx::st_hit) */
   uint64_t jobs_done;    /* jobs finished */
   double seconds;        /* wall time of the solve, device work included */
+  uint64_t tt_entry_reads; /* 16-byte TT entries read by probes */
+  uint64_t tt_stores;    /* TT stores (each rewrites a 64-byte tag line and one entry) */
 } ddx_stats;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -137,8 +140,8 @@ uint64_t ddx_gpu_device_bytes(const ddx_gpu* h);
  * wave_kernel batch run (gpu/gpu_main.cu:61-152, 231-274). */
 int ddx_gpu_solve(ddx_gpu* h, const uint64_t* hands, int64_t n, const uint8_t* seeds,
                   uint8_t* out, ddx_stats* stats);
-/* Same with device-resident buffers (HIP device pointers), on HIP stream `stream` (NULL =
- * the null stream); returns after the work is queued. Then ddx_gpu_wait(). */
+/* Same with device-resident buffers (HIP device pointers), on HIP stream `stream` (NULL = a
+ * non-blocking stream of the handle); returns after the work is queued. Then ddx_gpu_wait(). */
 int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t n, const uint8_t* d_seeds,
                         uint8_t* d_out, void* stream);
 /* Waits for the last async solve and fills stats. */
@@ -146,6 +149,8 @@ int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats);
 /* Asks a running solve to stop soon (reference TIME_LIMIT stop flag, gpu/gpu_main.cu:90,264);
  * callable from another thread. */
 void ddx_gpu_request_stop(ddx_gpu* h);
+/* Jobs finished so far by the running or last solve (callable while it runs). */
+int ddx_gpu_progress(ddx_gpu* h, uint64_t* jobs_done);
 /* Duration in ms of the last solve kernel, from HIP events on its stream. */
 double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
 /* Per-stage counters of the last solves (config.profile = 1), per wave64: out32[0..7] rounds by
@@ -154,6 +159,29 @@ double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
  * out32[16..22] where ENTER decided nodes (trivial, last trick, TT, quick tricks, later tricks,
  * 2nd-hand quick tricks, moves), out32[24..31] shader-clock cycles by kind. reset != 0 clears. */
 int ddx_gpu_profile(ddx_gpu* h, uint64_t* out32, int reset);
+
+/* ---- streaming solver (MI355X) --------------------------------------------------------- */
+
+/* One persistent kernel fed through `ring_slots` pinned host slots; each deal's table is
+ * published as soon as it is solved, in any order. Replaces dd_stream's kernel and ring
+ * (gpu/stream_main.cu:38-114, 146-164). cfg->jobs_per_deal 20 keeps more lanes busy than 5
+ * when ring_slots * jobs_per_deal is near the lane count. */
+typedef struct ddx_stream ddx_stream;
+int ddx_stream_open(const ddx_gpu_config* cfg, int ring_slots, ddx_stream** out);
+/* Queues one deal (hands N,E,S,W; seeds20 = NS-trick seeds [strain][declarer] for the seeded
+ * modes, else NULL) under a caller id. Returns DDX_OK, or DDX_E_FULL when every slot is in
+ * use (poll, then push again). Safe to call from one producer thread while another polls. */
+int ddx_stream_push(ddx_stream* s, const uint64_t hands[4], const uint8_t* seeds20, int64_t id);
+/* Takes up to `max` finished deals: ids[i] and tables[20*i] (the 28-byte output record of
+ * dd_stream, gpu/stream_main.cu:4-8, is id + table). Never blocks; *got = how many. */
+int ddx_stream_poll(ddx_stream* s, int max, int64_t* ids, uint8_t* tables, int* got);
+/* Deals pushed and not yet polled. */
+int64_t ddx_stream_in_flight(ddx_stream* s);
+/* No more pushes: the kernel ends once every queued deal is solved. */
+int ddx_stream_finish(ddx_stream* s);
+/* Stops the kernel (unsolved deals are dropped unless ddx_stream_finish was called and all
+ * results were polled), waits for it and frees everything. */
+int ddx_stream_close(ddx_stream* s, ddx_stats* stats);
 
 #ifdef __cplusplus
 }

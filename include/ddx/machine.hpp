@@ -56,7 +56,7 @@ struct Lane {
 DDX_FN void lane_init(Lane& L, TTView tt) {
   L.c.tt = tt;
   L.c.gen = 0;
-  L.c.nodes = L.c.probes = L.c.hits = 0;
+  L.c.nodes = L.c.probes = L.c.hits = L.c.ent_reads = L.c.stores = 0;
   L.depth = 0;
   L.stage = ST_JOB;
   L.v = 0;

@@ -109,6 +109,7 @@ struct Ctx {
   u32 gen;  // 1..63; tags of other generations count as empty
   u64 nodes;
   u64 probes, hits;
+  u64 ent_reads, stores;  // TT entries read by probes; TT stores (algorithmic HBM bytes)
 };
 
 // New generation for a new solve; after 63 solves every tag of the TT is cleared
@@ -230,7 +231,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
   int hint = -1;
   if (!DDX_NO_TT) {
-    int hit = tt_probe(c.tt, key, need, rel, hint);
+    int hit = tt_probe(c.tt, key, need, rel, hint, c.ent_reads);
     c.probes++;
     c.hits += hit >= 0;
     exit_at = EX_TT;
@@ -248,6 +249,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (decided) {
     const bool val = ns_lead ? q != 0 : q == 0;
     tt_store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+    c.stores++;
     rel = wr;
     exit_at = EX_QT;
     return val ? 1 : 0;
@@ -258,6 +260,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (!DDX_NO_LT && lt) {
     const bool val = !ns_lead;
     tt_store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+    c.stores++;
     rel = wr;
     exit_at = EX_LT;
     return val ? 1 : 0;
@@ -378,6 +381,7 @@ DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
     if (f.result) lb = f.need;
     else ub = f.need - 1;
     tt_store(c.tt, key, acc, lb, ub, f.best, c.gen);
+    c.stores++;
   }
   return acc;
 }

@@ -153,7 +153,7 @@ DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, by masks (
 // cards of the entry), -1 = unknown; then `hint` may get a stored best card. Matching tags
 // are visited newest first; a later matching entry's best card replaces an earlier one
 // (reference dd_tt_probe, src/dd.h:458-493).
-DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint) {
+DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint, u64& ent_reads) {
   TagLine L;
   load_line(tt, k.bucket, L);
   u32 m = match_mask(L, k.tag);
@@ -162,6 +162,7 @@ DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hin
     int pos = __builtin_ctz(m);
     m &= m - 1;
     Entry e = base[(tag_at(L, pos) >> 8) & 15];
+    ent_reads++;
     if ((e.w0 & kQMask) != k.q) continue;
     u64 r;
     if (!entry_matches(k, e.w0, e.w1, r)) continue;

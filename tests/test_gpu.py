@@ -1,0 +1,141 @@
+"""GPU parity: the MI355X kernels through the C ABI vs the golden vectors and the oracle."""
+import os
+import threading
+import time
+
+import numpy as np
+import pytest
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return golden("c1_endings.npz")
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_c1_endings_all_schedules(ddx, c1, schedule):
+    """4,096 8-card endings (C1), TT_LOG2 10: all 81,920 answers equal the reference's."""
+    log("c1 endings, schedule", schedule)
+    with ddx.GpuSolver(lanes=8192, tt_log2=10, schedule=schedule) as g:
+        out, st = g.solve(c1["hands"])
+    assert np.array_equal(out, c1["tables"])
+    assert st.jobs_done == 5 * len(c1["hands"]) and st.nodes > 0
+
+
+def test_per_lead_jobs(ddx, c1):
+    with ddx.GpuSolver(lanes=16384, tt_log2=10, jobs_per_deal=20) as g:
+        out, _ = g.solve(c1["hands"][:2048])
+    assert np.array_equal(out, c1["tables"][:2048])
+
+
+@pytest.mark.parametrize("mode", ["step", "cert", "certp"])
+def test_seed_modes_exact(ddx, c1, mode):
+    """Seeded searches (noisy seeds, even random ones) give the exact tables."""
+    m = {"step": ddx.SEED_STEP, "cert": ddx.SEED_CERT, "certp": ddx.SEED_CERTP}[mode]
+    rng = np.random.default_rng(11)
+    seeds = rng.integers(0, 14, (2048, 20)).astype(np.uint8)
+    seeds[:1024] = c1["seeds"][:1024]
+    with ddx.GpuSolver(lanes=8192, tt_log2=10, seed_mode=m) as g:
+        out, _ = g.solve(c1["hands"][:2048], seeds=None if mode == "certp" else seeds)
+    assert np.array_equal(out, c1["tables"][:2048])
+
+
+def test_endings_vs_oracle_mixed_sizes(ddx, oracle):
+    """Endings of 1..9 cards in one batch (uneven jobs) vs the oracle."""
+    hands = np.concatenate([ddx.random_deals(900 + k, 60, k) for k in range(1, 10)])
+    want = oracle.tables(hands)
+    with ddx.GpuSolver(lanes=4096, tt_log2=9, schedule=2) as g:
+        out, _ = g.solve(hands)
+    assert np.array_equal(out, want)
+
+
+def test_full_deals_vs_dds_tables(ddx, oracle):
+    """Full 13-card deals of the reference's DDS fixture, one job per lead."""
+    hands, tables = oracle.pgx(golden("dds_4000.npy")[:, :2, :])
+    t0 = time.time()
+    with ddx.GpuSolver(lanes=256, tt_log2=10, jobs_per_deal=20) as g:
+        out, st = g.solve(hands)
+    log("2 full deals: %.1fs, %.3g nodes" % (time.time() - t0, st.nodes))
+    assert np.array_equal(out, tables)
+
+
+# The eight bench-stream deals with the fewest reference nodes (a full deal's cost spans
+# 3e4 to 1e9 nodes; these keep the test within seconds).
+EASY_FULL = [1386, 707, 1081, 1897, 1468, 1032, 356, 1069]
+
+
+def test_full_deals_vs_reference_fixture(ddx):
+    """Deals of the bench stream vs the reference solver's tables."""
+    g = golden("full_2000.npz")
+    with ddx.GpuSolver(lanes=256, tt_log2=10, jobs_per_deal=20) as s:
+        out, _ = s.solve(g["hands"][EASY_FULL])
+    assert np.array_equal(out, g["tables"][EASY_FULL])
+
+
+def test_repeated_calls_keep_generations_apart(ddx, oracle):
+    """TT generations persist across calls: entries of an earlier batch never match."""
+    a = ddx.random_deals(71, 300, 6)
+    b = ddx.random_deals(72, 300, 6)
+    wa, wb = oracle.tables(a), oracle.tables(b)
+    with ddx.GpuSolver(lanes=1024, tt_log2=9) as g:
+        for _ in range(14):  # > 63 generations per lane: exercises the tag clear
+            assert np.array_equal(g.solve(a)[0], wa)
+            assert np.array_equal(g.solve(b)[0], wb)
+
+
+def test_stop_leaves_unsolved_marked(ddx, oracle):
+    """A stop request ends the run; unfinished answers are 0xFF, finished ones exact."""
+    hands = ddx.random_deals(81, 3000, 10)
+    with ddx.GpuSolver(lanes=1024, tt_log2=10) as g:
+        timer = threading.Timer(1.5, g.request_stop)
+        timer.start()
+        out, st = g.solve(hands, allow_stop=True)
+        timer.cancel()
+    done = (out != 255).all(axis=1)
+    assert 0 < st.jobs_done < 5 * len(hands)
+    want = oracle.tables(hands[done][:200])
+    assert np.array_equal(out[done][:200], want)
+    # answers of a strain are written lead by lead: every byte is either final or 0xFF
+    part = out[~done]
+    assert ((part == 255) | (part <= 13)).all()
+
+
+def test_stream_matches_batch(ddx, c1):
+    """The persistent streaming kernel: all deals come back, tables exact, ids intact."""
+    n = 1500
+    hands = c1["hands"][:n]
+    got = {}
+    with ddx.Stream(lanes=4096, tt_log2=10, jobs_per_deal=20, ring_slots=256) as s:
+        i = 0
+        t0 = time.time()
+        while len(got) < n and time.time() - t0 < 60:
+            while i < n and s.push(hands[i], i):
+                i += 1
+            ids, tabs = s.poll()
+            for k, t in zip(ids, tabs):
+                got[int(k)] = t
+            if i == n:
+                s.finish()
+            time.sleep(0.001)
+        st = s.close()
+    assert len(got) == n
+    out = np.stack([got[k] for k in range(n)])
+    assert np.array_equal(out, c1["tables"][:n])
+    assert st.jobs_done == 20 * n
+
+
+def test_bad_configs_fail_cleanly(ddx):
+    with pytest.raises(ddx.DdxError):
+        ddx.GpuSolver(lanes=100)
+    with pytest.raises(ddx.DdxError):
+        ddx.GpuSolver(tt_log2=8)
+    with ddx.GpuSolver(lanes=256) as g:
+        with pytest.raises(ddx.DdxError):
+            g.solve(np.array([[1, 1, 2, 4]], np.uint64))
