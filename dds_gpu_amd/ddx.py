@@ -31,7 +31,8 @@ class Stats(ctypes.Structure):
 class GpuConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int64), ("tt_log2", ctypes.c_int),
                 ("jobs_per_deal", ctypes.c_int), ("schedule", ctypes.c_int),
-                ("seed_mode", ctypes.c_int), ("block", ctypes.c_int), ("profile", ctypes.c_int)]
+                ("seed_mode", ctypes.c_int), ("block", ctypes.c_int), ("profile", ctypes.c_int),
+                ("team", ctypes.c_int)]
 
 
 class DdxError(RuntimeError):
@@ -83,6 +84,16 @@ def lib():
         L.ddx_stream_in_flight.restype = I64
         L.ddx_stream_finish.argtypes = [P]
         L.ddx_stream_close.argtypes = [P, P]
+        L.ddx_mlp_create.argtypes = [I, U64, P]
+        L.ddx_mlp_destroy.argtypes = [P]
+        L.ddx_mlp_param_count.restype = I64
+        L.ddx_mlp_params.argtypes = [P, P, I]
+        L.ddx_mlp_grads.argtypes = [P, P, P, I64, P, P]
+        L.ddx_mlp_train.argtypes = [P, P, P, I64, I, I, ctypes.c_float, U64, P]
+        L.ddx_mlp_predict.argtypes = [P, P, I64, P, P]
+        L.ddx_mlp_predict_host.argtypes = [P, P, I64, P]
+        L.ddx_mlp_save.argtypes = [P, ctypes.c_char_p]
+        L.ddx_mlp_load.argtypes = [P, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -179,12 +190,12 @@ class GpuSolver:
     """A set of solver lanes on one GPU (ddx_gpu_open / ddx_gpu_solve)."""
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=5, schedule=SCHED_CHAIN,
-                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0):
+                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
         cfg.jobs_per_deal, cfg.schedule, cfg.seed_mode = jobs_per_deal, schedule, seed_mode
-        cfg.block, cfg.profile = block, int(profile)
+        cfg.block, cfg.profile, cfg.team = block, int(profile), team
         self.cfg = cfg
         self.h = ctypes.c_void_p()
         _check(lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))
@@ -306,6 +317,73 @@ class Stream:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TrickNet:
+    """The trick-count MLP on the GPU (ddx_mlp_*)."""
+
+    def __init__(self, seed=1, device=0, path=None):
+        self.h = ctypes.c_void_p()
+        _check(lib().ddx_mlp_create(device, seed, ctypes.byref(self.h)))
+        if path:
+            self.load(path)
+
+    @staticmethod
+    def param_count():
+        return lib().ddx_mlp_param_count()
+
+    def params(self):
+        p = np.zeros(self.param_count(), np.float32)
+        _check(lib().ddx_mlp_params(self.h, _ptr(p), 1))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().ddx_mlp_params(self.h, _ptr(p), 0))
+
+    def grads(self, hands, tables):
+        hands = np.ascontiguousarray(hands, np.uint64)
+        tables = np.ascontiguousarray(tables, np.uint8)
+        g = np.zeros(self.param_count(), np.float32)
+        la = np.zeros(2, np.float32)
+        _check(lib().ddx_mlp_grads(self.h, _ptr(hands), _ptr(tables), len(hands), _ptr(g), _ptr(la)))
+        return g, float(la[0]), float(la[1])
+
+    def train(self, hands, tables, epochs=1, batch_deals=1024, lr=1e-3, shuffle_seed=0):
+        hands = np.ascontiguousarray(hands, np.uint64)
+        tables = np.ascontiguousarray(tables, np.uint8)
+        la = np.zeros(2, np.float32)
+        _check(lib().ddx_mlp_train(self.h, _ptr(hands), _ptr(tables), len(hands), epochs, batch_deals,
+                                   lr, shuffle_seed, _ptr(la)))
+        return float(la[0]), float(la[1])
+
+    def predict(self, hands):
+        """NS-trick seeds (n, 20) [strain][declarer]."""
+        hands = np.ascontiguousarray(hands, np.uint64)
+        out = np.zeros((len(hands), 20), np.uint8)
+        _check(lib().ddx_mlp_predict_host(self.h, _ptr(hands), len(hands), _ptr(out)))
+        return out
+
+    def predict_device(self, d_hands, n, d_seeds, stream=0):
+        _check(lib().ddx_mlp_predict(self.h, ctypes.c_void_p(d_hands), n, ctypes.c_void_p(d_seeds),
+                                     ctypes.c_void_p(stream) if stream else None))
+
+    def save(self, path):
+        _check(lib().ddx_mlp_save(self.h, path.encode()))
+
+    def load(self, path):
+        _check(lib().ddx_mlp_load(self.h, path.encode()))
+
+    def close(self):
+        if self.h:
+            lib().ddx_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -116,6 +116,9 @@ typedef struct ddx_gpu_config {
   int seed_mode;        /* DDX_SEED_* */
   int block;            /* threads per block (multiple of 64; default 256) */
   int profile;          /* 1: collect per-stage counters (slower) */
+  int team;             /* lanes per search: 1 (one search per lane, as the reference) or 16
+                           (a team of 16 lanes per search: lanes/16 searches, each with its own
+                           TT, an order of magnitude faster per search) */
 } ddx_gpu_config;
 
 enum {
@@ -126,7 +129,7 @@ enum {
 
 typedef struct ddx_gpu ddx_gpu;
 
-/* Default configuration (device 0, 65536 lanes, TT_LOG2 10, 5 jobs per deal). */
+/* Default configuration (device 0, 65536 lanes, TT_LOG2 10, 5 jobs per deal, team 1). */
 void ddx_gpu_default_config(ddx_gpu_config* cfg);
 /* Allocates the lanes' TTs and stacks on the device; replaces the reference's per-run
  * cudaMalloc of TT and DDWave state (gpu/gpu_main.cu:205-252). */
@@ -159,6 +162,37 @@ double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
  * out32[16..22] where ENTER decided nodes (trivial, last trick, TT, quick tricks, later tricks,
  * 2nd-hand quick tricks, moves), out32[24..31] shader-clock cycles by kind. reset != 0 clears. */
 int ddx_gpu_profile(ddx_gpu* h, uint64_t* out32, int reset);
+
+/* ---- trick-count net (MI355X) ------------------------------------------------------------ */
+
+/* An MLP predicting NS tricks per (deal, strain, leader): 217 inputs (card owners, strain,
+ * leader) -> 256 ReLU -> 256 ReLU -> 14 classes, BF16 weights and activations with FP32
+ * accumulation on the matrix cores, FP32 master weights and Adam. Its predictions are the seeds
+ * of DDX_SEED_CERT. No counterpart in the reference (its dd_solve_strain_guess takes a guess,
+ * src/dd.h:915-924, but nothing produces one). */
+typedef struct ddx_mlp ddx_mlp;
+int ddx_mlp_create(int device, uint64_t init_seed, ddx_mlp** out);
+void ddx_mlp_destroy(ddx_mlp* h);
+/* Length of the FP32 parameter vector (W1t[256][224], b1[256], W2t[256][256], b2[256],
+ * W3t[16][256], b3[16]; padding entries stay zero). */
+int64_t ddx_mlp_param_count(void);
+/* get != 0: copy the parameters out to buf; get == 0: load them from buf. */
+int ddx_mlp_params(ddx_mlp* h, float* buf, int get);
+/* Gradients of the mean cross-entropy over the 20 samples of each of n deals (host buffers:
+ * hands[4n], tables[20n] as declarer tricks); loss_acc[0] = mean loss, [1] = accuracy. */
+int ddx_mlp_grads(ddx_mlp* h, const uint64_t* hands, const uint8_t* tables, int64_t n, float* grads,
+                  float* loss_acc);
+/* Adam training, `epochs` passes over n deals in shuffled batches of batch_deals; loss_acc gets
+ * the last epoch's mean loss and accuracy. */
+int ddx_mlp_train(ddx_mlp* h, const uint64_t* hands, const uint8_t* tables, int64_t n, int epochs,
+                  int batch_deals, float lr, uint64_t shuffle_seed, float* loss_acc);
+/* Seeds (most likely NS tricks) for n deals: device buffers d_hands[4n] -> d_seeds[20n], ordered
+ * on HIP stream `stream` (NULL: synchronous). */
+int ddx_mlp_predict(ddx_mlp* h, const uint64_t* d_hands, int64_t n, uint8_t* d_seeds, void* stream);
+/* Same with host buffers. */
+int ddx_mlp_predict_host(ddx_mlp* h, const uint64_t* hands, int64_t n, uint8_t* seeds);
+int ddx_mlp_save(ddx_mlp* h, const char* path);
+int ddx_mlp_load(ddx_mlp* h, const char* path);
 
 /* ---- streaming solver (MI355X) --------------------------------------------------------- */
 

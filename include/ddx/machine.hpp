@@ -81,6 +81,7 @@ DDX_FN void start_search(Lane& L) {
 }
 
 // Loads job `job` of the batch into the lane (reference dd_wave_step JOB, src/dd_wave.h:72-90).
+template <class X = SeqOps>
 DDX_FN void lane_load_job(Lane& L, const Batch& B, long long job) {
   const int jpd = B.jobs_per_deal;
   L.deal = job / jpd;
@@ -92,7 +93,7 @@ DDX_FN void lane_load_job(Lane& L, const Batch& B, long long job) {
   L.decl = jpd == 5 ? 0 : sub % 4;
   L.last_decl = jpd == 5 ? 3 : L.decl;
   L.p.trump = strain_trump(L.strain);
-  next_generation(L.c);
+  next_generation<X>(L.c);
   L.prev_ns = 7;
   L.stage = ST_ROOT;
 }
@@ -144,11 +145,11 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
 
 // ENTER: enter the node at the current position; `st` is the lane's frame stack
 // (st.store(i, f) / st.load(i)).
-template <class Stack>
+template <class X = SeqOps, class Stack>
 DDX_FN void lane_enter(Lane& L, const Batch& B, Stack& st) {
   Frame f;
   u64 rel = 0;
-  const int e = enter(L.c, L.p, L.target, f, rel, L.exit_at);
+  const int e = enter<X>(L.c, L.p, L.target, f, rel, L.exit_at);
   if (e >= 0) {
     L.v = e;
     L.r = rel;
@@ -163,7 +164,7 @@ DDX_FN void lane_enter(Lane& L, const Batch& B, Stack& st) {
 }
 
 // The top frame finished with (L.v, L.r).
-template <class Stack>
+template <class X = SeqOps, class Stack>
 DDX_FN void lane_node_done(Lane& L, const Batch& B, Stack& st) {
   L.depth--;
   if (L.depth == 0) {
@@ -175,7 +176,7 @@ DDX_FN void lane_node_done(Lane& L, const Batch& B, Stack& st) {
 }
 
 // NEXT: play the next move of the top frame, or finish it.
-template <class Stack>
+template <class X = SeqOps, class Stack>
 DDX_FN void lane_next(Lane& L, const Batch& B, Stack& st) {
   const int card = next_move(L.top);
   if (card >= 0) {
@@ -184,14 +185,14 @@ DDX_FN void lane_next(Lane& L, const Batch& B, Stack& st) {
     return;
   }
   L.v = L.top.result;
-  L.r = finish(L.c, L.p, L.top);
-  lane_node_done(L, B, st);
+  L.r = finish<X>(L.c, L.p, L.top);
+  lane_node_done<X>(L, B, st);
 }
 
 // RET: the child of the top frame returned (L.v, L.r): undo its move; a result that reaches
 // the mover's goal cuts off, else collect its relevant cards and try the next move
 // (reference dd_search child_done, src/dd.h:840-854).
-template <class Stack>
+template <class X = SeqOps, class Stack>
 DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
   Frame& f = L.top;
   const int card = move_at(f, f.i);
@@ -200,8 +201,8 @@ DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
     f.result = L.v;
     f.acc = L.r;
     f.best = card;
-    L.r = finish(L.c, L.p, f);
-    lane_node_done(L, B, st);
+    L.r = finish<X>(L.c, L.p, f);
+    lane_node_done<X>(L, B, st);
     return;
   }
   f.acc |= L.r;
@@ -210,7 +211,7 @@ DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
 }
 
 // One stage of the machine. `take` returns the next job index (>= jobs: none left).
-template <class Stack, class Take>
+template <class X = SeqOps, class Stack, class Take>
 DDX_FN void lane_step(Lane& L, const Batch& B, Stack& st, Take take) {
   switch (L.stage) {
     case ST_JOB: {
@@ -219,20 +220,20 @@ DDX_FN void lane_step(Lane& L, const Batch& B, Stack& st, Take take) {
         L.stage = ST_EXIT;
         return;
       }
-      lane_load_job(L, B, job);
+      lane_load_job<X>(L, B, job);
       return;
     }
     case ST_ROOT:
       lane_root(L, B);
       return;
     case ST_ENTER:
-      lane_enter(L, B, st);
+      lane_enter<X>(L, B, st);
       return;
     case ST_NEXT:
-      lane_next(L, B, st);
+      lane_next<X>(L, B, st);
       return;
     case ST_RET:
-      lane_ret(L, B, st);
+      lane_ret<X>(L, B, st);
       return;
     default:
       return;

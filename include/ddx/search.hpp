@@ -112,19 +112,6 @@ struct Ctx {
   u64 ent_reads, stores;  // TT entries read by probes; TT stores (algorithmic HBM bytes)
 };
 
-// New generation for a new solve; after 63 solves every tag of the TT is cleared
-// (reference dd_next_gen, src/dd.h:864-874).
-DDX_FN void next_generation(Ctx& c) {
-  c.gen++;
-  if (c.gen >= (1u << kGenBits)) {
-    Q4 z = {0, 0, 0, 0};
-    Q4* t = (Q4*)c.tt.tags;
-    size_t n = ((size_t)c.tt.mask + 1) * (kWays / 4);
-    for (size_t i = 0; i < n; i++) t[i] = z;
-    c.gen = 1;
-  }
-}
-
 // ---------------------------------------------------------------------------------------
 // Move generation: one candidate per run of equivalent cards, best first by the DDS
 // weights; the TT's best card (or one equal to it) is favoured (reference dd_moves,
@@ -183,6 +170,38 @@ DDX_FN void gen_moves(const Pos& p, int seat, int hint, const Top& t, Frame& f) 
   f.n = n;
 }
 
+// The data-parallel pieces of a node, as a policy: SeqOps runs them on one thread (the CPU, and
+// the GPU kernel with one lane per search); the GPU team kernel supplies a version that spreads
+// them over the lanes of a team. Everything else in a node runs the same code either way.
+struct SeqOps {
+  DDX_M static int probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint, u64& ent_reads) {
+    return tt_probe(tt, k, need, rel, hint, ent_reads);
+  }
+  DDX_M static void store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, int best, u32 gen) {
+    tt_store(tt, k, rel, lb, ub, best, gen);
+  }
+  DDX_M static void moves(const Pos& p, int seat, int hint, const Top& t, Frame& f) {
+    gen_moves(p, seat, hint, t, f);
+  }
+  DDX_M static void clear_tags(const TTView& tt) {
+    Q4 z = {0, 0, 0, 0};
+    Q4* q = (Q4*)tt.tags;
+    const size_t n = ((size_t)tt.mask + 1) * (kWays / 4);
+    for (size_t i = 0; i < n; i++) q[i] = z;
+  }
+};
+
+// New generation for a new solve; after 63 solves every tag of the TT is cleared
+// (reference dd_next_gen, src/dd.h:864-874).
+template <class X = SeqOps>
+DDX_FN void next_generation(Ctx& c) {
+  c.gen++;
+  if (c.gen >= (1u << kGenBits)) {
+    X::clear_tags(c.tt);
+    c.gen = 1;
+  }
+}
+
 // Compile-time switches used only by the tests' "plain" build: no TT, no quick tricks, no
 // later tricks, no second-hand quick tricks, no small-card skip (reference DD_NO_*,
 // tests/check.sh:18).
@@ -209,6 +228,7 @@ enum { EX_TRIVIAL, EX_LAST, EX_TT, EX_QT, EX_LT, EX_QT2, EX_MOVES, EX_COUNT };
 // node is decided at once (`rel` = its relevant cards), else -1 with `f` holding the
 // ordered moves. Splits into the trick-start part (TT, quick tricks, later tricks) and the
 // rest so that the GPU can schedule them as separate stages.
+template <class X = SeqOps>
 DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int& exit_at) {
   c.nodes++;
   rel = 0;
@@ -231,7 +251,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
   int hint = -1;
   if (!DDX_NO_TT) {
-    int hit = tt_probe(c.tt, key, need, rel, hint, c.ent_reads);
+    int hit = X::probe(c.tt, key, need, rel, hint, c.ent_reads);
     c.probes++;
     c.hits += hit >= 0;
     exit_at = EX_TT;
@@ -248,7 +268,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (DDX_NO_QT) decided = false;
   if (decided) {
     const bool val = ns_lead ? q != 0 : q == 0;
-    tt_store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+    X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
     c.stores++;
     rel = wr;
     exit_at = EX_QT;
@@ -259,13 +279,13 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
                           : later_max(t, p.trump, p.leader, p.ns_won, p.left, target, wr);
   if (!DDX_NO_LT && lt) {
     const bool val = !ns_lead;
-    tt_store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+    X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
     c.stores++;
     rel = wr;
     exit_at = EX_LT;
     return val ? 1 : 0;
   }
-  gen_moves(p, p.leader, hint, t, f);
+  X::moves(p, p.leader, hint, t, f);
   f.seat = p.leader;
   f.result = (p.leader & 1) ? 1 : 0;  // the side to move has not yet reached its goal
   f.need = need;
@@ -279,6 +299,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   return -1;
 }
 
+template <class X = SeqOps>
 DDX_FN int enter_follow(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int& exit_at) {
   c.nodes++;
   Top t;
@@ -299,7 +320,7 @@ DDX_FN int enter_follow(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, in
     }
   }
   const int seat = (p.leader + p.nplayed) & 3;
-  gen_moves(p, seat, -1, t, f);
+  X::moves(p, seat, -1, t, f);
   f.seat = seat;
   f.result = (seat & 1) ? 1 : 0;
   f.need = 0;
@@ -313,9 +334,10 @@ DDX_FN int enter_follow(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, in
   return -1;
 }
 
+template <class X = SeqOps>
 DDX_FN int enter(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int& exit_at) {
-  return p.nplayed == 0 ? enter_lead(c, p, target, f, rel, exit_at)
-                        : enter_follow(c, p, target, f, rel, exit_at);
+  return p.nplayed == 0 ? enter_lead<X>(c, p, target, f, rel, exit_at)
+                        : enter_follow<X>(c, p, target, f, rel, exit_at);
 }
 
 // Plays `card` for the frame's seat; a fourth card settles the trick (reference dd_play,
@@ -359,6 +381,7 @@ DDX_FN void unplay(Pos& p, const Frame& f, int card, u64& rel) {
 // equivalent cards was tried, so when the lowest relevant card of a suit heads such a run
 // the rest of the run must be relevant too -- else a TT entry could match a position where
 // another hand's card splits the run.
+template <class X = SeqOps>
 DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
   u64 acc = f.acc;
   const u64 own = hand_of(p, f.seat);
@@ -380,7 +403,7 @@ DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
     int lb = f.lb, ub = f.ub;
     if (f.result) lb = f.need;
     else ub = f.need - 1;
-    tt_store(c.tt, key, acc, lb, ub, f.best, c.gen);
+    X::store(c.tt, key, acc, lb, ub, f.best, c.gen);
     c.stores++;
   }
   return acc;

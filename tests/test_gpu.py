@@ -19,20 +19,30 @@ def c1():
     return golden("c1_endings.npz")
 
 
+@pytest.mark.parametrize("team", [1, 16])
 @pytest.mark.parametrize("schedule", [0, 1, 2])
-def test_c1_endings_all_schedules(ddx, c1, schedule):
-    """4,096 8-card endings (C1), TT_LOG2 10: all 81,920 answers equal the reference's."""
-    log("c1 endings, schedule", schedule)
-    with ddx.GpuSolver(lanes=8192, tt_log2=10, schedule=schedule) as g:
+def test_c1_endings_all_schedules(ddx, c1, schedule, team):
+    """4,096 8-card endings (C1), TT_LOG2 10: all 81,920 answers equal the reference's, with one
+    lane per search and with 16-lane teams."""
+    log("c1 endings, schedule", schedule, "team", team)
+    with ddx.GpuSolver(lanes=8192 * team, tt_log2=10, schedule=schedule, team=team) as g:
         out, st = g.solve(c1["hands"])
     assert np.array_equal(out, c1["tables"])
     assert st.jobs_done == 5 * len(c1["hands"]) and st.nodes > 0
+    log("nodes gpu %d, reference one-context total %d" % (st.nodes, int(c1["nodes"].sum())))
 
 
 def test_per_lead_jobs(ddx, c1):
     with ddx.GpuSolver(lanes=16384, tt_log2=10, jobs_per_deal=20) as g:
         out, _ = g.solve(c1["hands"][:2048])
     assert np.array_equal(out, c1["tables"][:2048])
+
+
+def test_team_stream_of_mixed_endings(ddx, oracle):
+    hands = np.concatenate([ddx.random_deals(700 + k, 50, k) for k in range(1, 11)])
+    with ddx.GpuSolver(lanes=16384, tt_log2=9, team=16, jobs_per_deal=20) as g:
+        out, _ = g.solve(hands)
+    assert np.array_equal(out, oracle.tables(hands))
 
 
 @pytest.mark.parametrize("mode", ["step", "cert", "certp"])
@@ -60,7 +70,7 @@ def test_full_deals_vs_dds_tables(ddx, oracle):
     """Full 13-card deals of the reference's DDS fixture, one job per lead."""
     hands, tables = oracle.pgx(golden("dds_4000.npy")[:, :2, :])
     t0 = time.time()
-    with ddx.GpuSolver(lanes=256, tt_log2=10, jobs_per_deal=20) as g:
+    with ddx.GpuSolver(lanes=4096, tt_log2=12, jobs_per_deal=20, team=16) as g:
         out, st = g.solve(hands)
     log("2 full deals: %.1fs, %.3g nodes" % (time.time() - t0, st.nodes))
     assert np.array_equal(out, tables)
@@ -74,9 +84,12 @@ EASY_FULL = [1386, 707, 1081, 1897, 1468, 1032, 356, 1069]
 def test_full_deals_vs_reference_fixture(ddx):
     """Deals of the bench stream vs the reference solver's tables."""
     g = golden("full_2000.npz")
-    with ddx.GpuSolver(lanes=256, tt_log2=10, jobs_per_deal=20) as s:
-        out, _ = s.solve(g["hands"][EASY_FULL])
-    assert np.array_equal(out, g["tables"][EASY_FULL])
+    for team in (1, 16):
+        with ddx.GpuSolver(lanes=256 * team, tt_log2=10, jobs_per_deal=20, team=team) as s:
+            out, _ = s.solve(g["hands"][EASY_FULL])
+        assert np.array_equal(out, g["tables"][EASY_FULL])
+
+
 
 
 def test_repeated_calls_keep_generations_apart(ddx, oracle):
