@@ -10,6 +10,9 @@
 //   SCHEDULE    0 priority (the reference's), 1 chain (CHEAP_LOOP=1), 2 split (default)
 //   PER_LEAD=1  one job per lead (20 per deal) instead of per strain
 //   SEEDS=FILE  n x 20 uint8 NS-trick seeds for the seeded search (DDX_SEED_CERT)
+//   MODEL=FILE  a trained trick-count net (dd_train): its predictions seed DDX_SEED_CERT; the
+//               prediction time is counted in the reported time
+//   TEAM=16     sixteen lanes per search (THREADS / 16 searches)
 //   WAVE, STACK_KB  accepted for compatibility; the wavefront kernel is the only kernel and
 //               its frames live in HBM, so there is no stack limit to set
 #include <stdio.h>
@@ -68,9 +71,10 @@ int main(int argc, char** argv) {
   ddx_gpu_config cfg;
   ddx_gpu_default_config(&cfg);
   cfg.jobs_per_deal = getenv("PER_LEAD") && atoi(getenv("PER_LEAD")) ? 20 : 5;
+  cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
   const long long jobs = n * cfg.jobs_per_deal;
   long long threads = getenv("THREADS") ? atoll(getenv("THREADS")) : 32768;
-  if (threads > jobs) threads = jobs;
+  if (threads > jobs * cfg.team) threads = jobs * cfg.team;  // spare searches would find no job
   threads = (threads + 255) / 256 * 256;
   cfg.lanes = threads;
   cfg.tt_log2 = tt_log2;
@@ -87,6 +91,21 @@ int main(int argc, char** argv) {
     }
     fclose(f);
     cfg.seed_mode = DDX_SEED_CERT;
+  }
+  double predict_s = 0;
+  if (getenv("MODEL")) {
+    ddx_mlp* net = nullptr;
+    seeds.resize(n * 20);
+    const auto p0 = now_s();
+    if (ddx_mlp_create(0, 1, &net) != DDX_OK || ddx_mlp_load(net, getenv("MODEL")) != DDX_OK ||
+        ddx_mlp_predict_host(net, hands.data(), n, seeds.data()) != DDX_OK) {
+      fprintf(stderr, "%s\n", ddx_last_error());
+      return 1;
+    }
+    ddx_mlp_destroy(net);
+    predict_s = now_s() - p0;
+    cfg.seed_mode = DDX_SEED_CERT;
+    printf("learned seeds from %s: %.3f s\n", getenv("MODEL"), predict_s);
   }
   printf("%lld threads, ", threads);
   printf("%lld deals, %lld solves, TT %.0f KB/solve, %.2f GB total\n", n, jobs,
@@ -132,7 +151,7 @@ int main(int argc, char** argv) {
     ddx_gpu_close(h);
     return 1;
   }
-  const double sec = stats.seconds;
+  const double sec = stats.seconds + predict_s;
   if (stopped)
     printf("finished jobs: %llu in %.1f s = %.1f full-job deals/s\n", (unsigned long long)stats.jobs_done,
            sec, stats.jobs_done / (double)cfg.jobs_per_deal / sec);

@@ -52,26 +52,6 @@ struct KernelArgs {
   unsigned long long* prof;      // rounds [0..7], lanes [8..15], exits [16..22], cycles [24..31]
 };
 
-// One lane per search (the reference's mapping) or a team of 16 lanes per search.
-template <int TEAM>
-struct TeamOf;
-template <>
-struct TeamOf<1> {
-  using Ops = SeqOps;
-  __device__ static bool leader() { return true; }
-  __device__ static long long take(unsigned long long* next) { return (long long)atomicAdd(next, 1ull); }
-};
-template <>
-struct TeamOf<16> {
-  using Ops = Team16;
-  __device__ static bool leader() { return Team16::lane() == 0; }
-  __device__ static long long take(unsigned long long* next) {
-    long long j = 0;
-    if (Team16::lane() == 0) j = (long long)atomicAdd(next, 1ull);
-    return Team16::bcast(j, 0);
-  }
-};
-
 template <int SCHED, bool PROF, int TEAM>
 __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
   using TM = TeamOf<TEAM>;
@@ -124,7 +104,8 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
   };
 
   for (;;) {
-    if ((++rounds & 4095) == 0 && *a.stop) L.stage = ST_EXIT;
+    // The stop flag changes under the kernel: one lane reads it for the whole team.
+    if ((++rounds & 4095) == 0 && TM::bcast((int)*a.stop, 0)) L.stage = ST_EXIT;
     if (L.jobs_done != reported) {  // progress for the host (one atomic per finished job)
       if (TM::leader()) atomicAdd(a.counters + 5, (unsigned long long)(L.jobs_done - reported));
       reported = L.jobs_done;
@@ -275,6 +256,12 @@ struct ddx_gpu {
   hipStream_t stream = nullptr;  // stream of the last launch
   hipStream_t own = nullptr;     // non-blocking stream for calls that pass none
   hipStream_t side = nullptr;    // non-blocking stream for progress reads
+  hipGraphExec_t gexec = nullptr;  // captured batch (config.graph)
+  int64_t g_n = -1;
+  const void* g_hands = nullptr;
+  const void* g_out = nullptr;
+  const void* g_seeds = nullptr;
+  hipStream_t g_stream = nullptr;
   std::chrono::steady_clock::time_point t0;
   double last_ms = 0;
   int64_t last_jobs = 0;
@@ -291,6 +278,7 @@ extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
   cfg->block = 256;
   cfg->profile = 0;
   cfg->team = 1;
+  cfg->graph = 0;
 }
 
 static void free_all(ddx_gpu* h) {
@@ -302,6 +290,7 @@ static void free_all(ddx_gpu* h) {
   if (h->d_out) (void)hipFree(h->d_out);
   if (h->d_seeds) (void)hipFree(h->d_seeds);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->own) (void)hipStreamDestroy(h->own);
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -382,6 +371,32 @@ static void launch_solve(const KernelArgs& a, int grid, int block, int team, hip
   else hipLaunchKernelGGL((solve_kernel<S, P, 1>), dim3(grid), dim3(block), 0, s, a);
 }
 
+// Queues one batch on stream s: clear the counters, mark every answer unsolved (0xFF), run the
+// solver. Kernels only (no memset nodes), so the sequence can be captured in a graph.
+static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t s) {
+  int rc;
+  if ((rc = launch_fill(h->counters, 64, 0, s)) != DDX_OK) return rc;
+  if (n) {
+    int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, a.out, (size_t)(20 * n),
+                       (uint8_t)0xFF);
+    HIPCHECK(hipGetLastError());
+  }
+  const int block = h->cfg.block;
+  const int grid = (int)(h->cfg.lanes / block);
+  const bool P = h->cfg.profile != 0;
+  switch (h->cfg.schedule * 2 + (P ? 1 : 0)) {
+    case 0: launch_solve<DDX_SCHED_PRIORITY, false>(a, grid, block, h->cfg.team, s); break;
+    case 1: launch_solve<DDX_SCHED_PRIORITY, true>(a, grid, block, h->cfg.team, s); break;
+    case 2: launch_solve<DDX_SCHED_CHAIN, false>(a, grid, block, h->cfg.team, s); break;
+    case 3: launch_solve<DDX_SCHED_CHAIN, true>(a, grid, block, h->cfg.team, s); break;
+    case 4: launch_solve<DDX_SCHED_SPLIT, false>(a, grid, block, h->cfg.team, s); break;
+    default: launch_solve<DDX_SCHED_SPLIT, true>(a, grid, block, h->cfg.team, s); break;
+  }
+  HIPCHECK(hipGetLastError());
+  return DDX_OK;
+}
+
 extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t n,
                                    const uint8_t* d_seeds, uint8_t* d_out, void* stream) {
   if (!h || n < 0 || (n && (!d_hands || !d_out)) ||
@@ -395,14 +410,6 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
   __atomic_store_n(h->h_stop, 0, __ATOMIC_SEQ_CST);
   const long long jobs = (long long)n * h->cfg.jobs_per_deal;
   h->last_jobs = jobs;
-  int rc;
-  if ((rc = launch_fill(h->counters, 64, 0, s)) != DDX_OK) return rc;
-  if (n) {
-    int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
-    hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, d_out, (size_t)(20 * n),
-                       (uint8_t)0xFF);
-    HIPCHECK(hipGetLastError());
-  }
   KernelArgs a;
   a.hands = d_hands;
   a.out = d_out;
@@ -414,20 +421,34 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
   a.counters = h->counters;
   a.stop = h->d_stop;
   a.prof = h->prof;
-  const int block = h->cfg.block;
-  const int grid = (int)(h->cfg.lanes / block);
   h->t0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(h->ev0, s));
-  const bool P = h->cfg.profile != 0;
-  switch (h->cfg.schedule * 2 + (P ? 1 : 0)) {
-    case 0: launch_solve<DDX_SCHED_PRIORITY, false>(a, grid, block, h->cfg.team, s); break;
-    case 1: launch_solve<DDX_SCHED_PRIORITY, true>(a, grid, block, h->cfg.team, s); break;
-    case 2: launch_solve<DDX_SCHED_CHAIN, false>(a, grid, block, h->cfg.team, s); break;
-    case 3: launch_solve<DDX_SCHED_CHAIN, true>(a, grid, block, h->cfg.team, s); break;
-    case 4: launch_solve<DDX_SCHED_SPLIT, false>(a, grid, block, h->cfg.team, s); break;
-    default: launch_solve<DDX_SCHED_SPLIT, true>(a, grid, block, h->cfg.team, s); break;
+  if (!h->cfg.graph) {
+    const int rc = enqueue_solve(h, a, n, s);
+    if (rc != DDX_OK) return rc;
+  } else {
+    // One graph per (buffers, size, stream): fill kernels + solve kernel, replayed on reuse.
+    if (!h->gexec || h->g_n != n || h->g_hands != d_hands || h->g_out != d_out ||
+        h->g_seeds != d_seeds || h->g_stream != s) {
+      if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+      h->gexec = nullptr;
+      hipGraph_t graph;
+      HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_solve(h, a, n, s);
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (rc != DDX_OK) return rc;
+      HIPCHECK(e);
+      e = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+      h->g_n = n;
+      h->g_hands = d_hands;
+      h->g_out = d_out;
+      h->g_seeds = d_seeds;
+      h->g_stream = s;
+    }
+    HIPCHECK(hipGraphLaunch(h->gexec, s));
   }
-  HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(h->ev1, s));
   return DDX_OK;
 }

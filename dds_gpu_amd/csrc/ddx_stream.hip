@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "ddx_lanes.hpp"
+#include "ddx_team.hpp"
 
 using namespace ddx;
 
@@ -53,8 +54,14 @@ __device__ __forceinline__ long long load_acquire_sys(volatile long long* p) {
   return __hip_atomic_load((long long*)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// TEAM = 1: one lane per search; 16: a 16-lane team per search. Values the host changes while
+// the kernel runs (produced, eof, stop) are read by one lane and broadcast to its team, so the
+// lanes of a team never disagree.
+template <int TEAM>
 __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
-  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  using TM = TeamOf<TEAM>;
+  using X = typename TM::Ops;
+  const long long lane = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / TEAM;
   Lane L;
   GpuStack st;
   lane_attach(a.lanes, lane, L, st);
@@ -64,33 +71,36 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   u32 published = 0;
   unsigned rounds = 0;
   for (;;) {
-    if ((++rounds & 1023) == 0 && a.ctl->stop) L.stage = ST_EXIT;
-    // Cheap stages until the lane must enter a node, needs a job, or is done.
+    if ((++rounds & 1023) == 0 && TM::bcast((int)a.ctl->stop, 0)) L.stage = ST_EXIT;
+    // Cheap stages until the search must enter a node, needs a job, or is done.
     while (L.stage == ST_ROOT || L.stage == ST_NEXT || L.stage == ST_RET) {
       if (L.stage == ST_ROOT) lane_root(L, B);
-      else if (L.stage == ST_NEXT) lane_next(L, B, st);
-      else lane_ret(L, B, st);
+      else if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
+      else lane_ret<X>(L, B, st);
     }
     if (L.jobs_done != published) {  // a job just finished: count it, maybe publish the deal
       published = L.jobs_done;
       const long long slot = L.deal;
-      __threadfence_system();
-      if (atomicAdd(&a.deal_done[slot], 1u) == (u32)jpd - 1) {
-        atomicExch(&a.deal_done[slot], 0u);
+      if (TM::leader()) {
         __threadfence_system();
-        const unsigned long long pos = atomicAdd(&a.counters[1], 1ull);
-        __hip_atomic_store(&a.comp[pos % a.ring], (u32)slot + 1, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        if (atomicAdd(&a.deal_done[slot], 1u) == (u32)jpd - 1) {
+          atomicExch(&a.deal_done[slot], 0u);
+          __threadfence_system();
+          const unsigned long long pos = atomicAdd(&a.counters[1], 1ull);
+          __hip_atomic_store(&a.comp[pos % a.ring], (u32)slot + 1, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_SYSTEM);
+        }
       }
     }
     if (L.stage == ST_JOB) {
-      if (pending < 0) pending = (long long)atomicAdd(&a.counters[0], 1ull);
+      if (pending < 0) pending = TM::take(&a.counters[0]);
       const long long p = pending / jpd;
-      if (p < load_acquire_sys(&a.ctl->produced)) {
+      const long long produced = TM::bcast(load_acquire_sys(&a.ctl->produced), 0);
+      if (p < produced) {
         const long long slot = a.work[p % a.work_ring];
-        lane_load_job(L, B, slot * jpd + pending % jpd);  // to ROOT; runs next round
+        lane_load_job<X>(L, B, slot * jpd + pending % jpd);  // to ROOT; runs next round
         pending = -1;
-      } else if (a.ctl->eof || a.ctl->stop) {
+      } else if (TM::bcast((int)(a.ctl->eof || a.ctl->stop), 0)) {
         L.stage = ST_EXIT;
       }
     }
@@ -98,15 +108,16 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
     // and __any see all 64 lanes.
     if (__all(L.stage == ST_EXIT)) break;
     if (__any(L.stage == ST_ENTER && L.p.nplayed != 0)) {
-      if (L.stage == ST_ENTER && L.p.nplayed != 0) lane_enter(L, B, st);
+      if (L.stage == ST_ENTER && L.p.nplayed != 0) lane_enter<X>(L, B, st);
       continue;
     }
     if (__any(L.stage == ST_ENTER)) {
-      if (L.stage == ST_ENTER) lane_enter(L, B, st);
+      if (L.stage == ST_ENTER) lane_enter<X>(L, B, st);
       continue;
     }
     if (!__any(L.stage == ST_ROOT)) __builtin_amdgcn_s_sleep(64);  // all waiting for input
   }
+  if (!TM::leader()) return;
   a.lanes.gen[lane] = L.c.gen;
   atomicAdd(&a.counters[2], (unsigned long long)L.c.nodes);
   atomicAdd(&a.counters[3], (unsigned long long)L.c.probes);
@@ -176,7 +187,8 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   if (cfg_in) cfg = *cfg_in;
   else ddx_gpu_default_config(&cfg);
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 256 ||
-      (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3) {
+      (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3 ||
+      (cfg.team != 1 && cfg.team != 16)) {
     ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, 20], "
                   "jobs_per_deal 5 or 20)");
     return DDX_E_ARG;
@@ -193,7 +205,7 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   s->ring = ring_slots;
   s->work_ring = 1;
   while (s->work_ring < 4 * s->ring) s->work_ring <<= 1;
-  int rc = lanes_alloc(s->lanes, cfg.lanes, cfg.tt_log2);
+  int rc = lanes_alloc(s->lanes, cfg.lanes / cfg.team, cfg.tt_log2);
   if (rc != DDX_OK) {
     delete s;
     return rc;
@@ -237,7 +249,10 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   a.seed_mode = cfg.seed_mode;
   if (!a.hands || !a.seeds || !a.out || !a.work || !a.comp || !a.ctl) return fail("mapped views");
   s->t0 = std::chrono::steady_clock::now();
-  hipLaunchKernelGGL(stream_kernel, dim3((unsigned)(cfg.lanes / 256)), dim3(256), 0, s->stream, a);
+  if (cfg.team == 16)
+    hipLaunchKernelGGL(stream_kernel<16>, dim3((unsigned)(cfg.lanes / 256)), dim3(256), 0, s->stream, a);
+  else
+    hipLaunchKernelGGL(stream_kernel<1>, dim3((unsigned)(cfg.lanes / 256)), dim3(256), 0, s->stream, a);
   if (hipGetLastError() != hipSuccess) {
     ddx_set_error("ddx_stream_open: kernel launch failed");
     stream_free(s);

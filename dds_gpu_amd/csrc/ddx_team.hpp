@@ -198,4 +198,29 @@ struct Team16 {
   }
 };
 
+// One lane per search (the reference's mapping) or a team of 16 lanes per search: the ops and
+// the team-wide job counter / broadcast used by the kernels.
+template <int TEAM>
+struct TeamOf;
+template <>
+struct TeamOf<1> {
+  using Ops = SeqOps;
+  __device__ static bool leader() { return true; }
+  template <class V>
+  __device__ static V bcast(V v, int) { return v; }
+  __device__ static long long take(unsigned long long* next) { return (long long)atomicAdd(next, 1ull); }
+};
+template <>
+struct TeamOf<16> {
+  using Ops = Team16;
+  __device__ static bool leader() { return Team16::lane() == 0; }
+  __device__ static int bcast(int v, int src) { return Team16::bcast(v, src); }
+  __device__ static long long bcast(long long v, int src) { return Team16::bcast(v, src); }
+  __device__ static long long take(unsigned long long* next) {
+    long long j = 0;
+    if (Team16::lane() == 0) j = (long long)atomicAdd(next, 1ull);
+    return Team16::bcast(j, 0);
+  }
+};
+
 }  // namespace ddx

@@ -32,7 +32,7 @@ class GpuConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int64), ("tt_log2", ctypes.c_int),
                 ("jobs_per_deal", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("seed_mode", ctypes.c_int), ("block", ctypes.c_int), ("profile", ctypes.c_int),
-                ("team", ctypes.c_int)]
+                ("team", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class DdxError(RuntimeError):
@@ -190,12 +190,12 @@ class GpuSolver:
     """A set of solver lanes on one GPU (ddx_gpu_open / ddx_gpu_solve)."""
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=5, schedule=SCHED_CHAIN,
-                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1):
+                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1, graph=False):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
         cfg.jobs_per_deal, cfg.schedule, cfg.seed_mode = jobs_per_deal, schedule, seed_mode
-        cfg.block, cfg.profile, cfg.team = block, int(profile), team
+        cfg.block, cfg.profile, cfg.team, cfg.graph = block, int(profile), team, int(graph)
         self.cfg = cfg
         self.h = ctypes.c_void_p()
         _check(lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))
@@ -275,10 +275,10 @@ class Stream:
     """The persistent streaming solver (ddx_stream_*): push deals, poll finished tables."""
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=20, seed_mode=SEED_PLAIN,
-                 ring_slots=4096, device=0):
+                 ring_slots=4096, device=0, team=1):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
-        cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
+        cfg.device, cfg.lanes, cfg.tt_log2, cfg.team = device, lanes, tt_log2, team
         cfg.jobs_per_deal, cfg.seed_mode = jobs_per_deal, seed_mode
         self.h = ctypes.c_void_p()
         _check(lib().ddx_stream_open(ctypes.byref(cfg), ring_slots, ctypes.byref(self.h)))

@@ -119,6 +119,8 @@ typedef struct ddx_gpu_config {
   int team;             /* lanes per search: 1 (one search per lane, as the reference) or 16
                            (a team of 16 lanes per search: lanes/16 searches, each with its own
                            TT, an order of magnitude faster per search) */
+  int graph;            /* 1: launch each batch as a captured HIP graph (clear + solve kernels),
+                           re-captured only when the buffers, size or stream change */
 } ddx_gpu_config;
 
 enum {

@@ -120,12 +120,13 @@ def test_stop_leaves_unsolved_marked(ddx, oracle):
     assert ((part == 255) | (part <= 13)).all()
 
 
-def test_stream_matches_batch(ddx, c1):
+@pytest.mark.parametrize("team,tt_log2", [(1, 10), (16, 12)])
+def test_stream_matches_batch(ddx, c1, team, tt_log2):
     """The persistent streaming kernel: all deals come back, tables exact, ids intact."""
     n = 1500
     hands = c1["hands"][:n]
     got = {}
-    with ddx.Stream(lanes=4096, tt_log2=10, jobs_per_deal=20, ring_slots=256) as s:
+    with ddx.Stream(lanes=4096, tt_log2=tt_log2, jobs_per_deal=20, ring_slots=256, team=team) as s:
         i = 0
         t0 = time.time()
         while len(got) < n and time.time() - t0 < 60:
@@ -152,3 +153,18 @@ def test_bad_configs_fail_cleanly(ddx):
     with ddx.GpuSolver(lanes=256) as g:
         with pytest.raises(ddx.DdxError):
             g.solve(np.array([[1, 1, 2, 4]], np.uint64))
+
+
+def test_graph_launch_replays_match_eager(ddx, c1, oracle):
+    """Batches launched as a captured HIP graph, replayed several times with eager work in
+    between: every replay's tables are exact."""
+    hs = [c1["hands"][i * 500:(i + 1) * 500] for i in range(3)]
+    with ddx.GpuSolver(lanes=4096, tt_log2=10, graph=True) as g, \
+            ddx.GpuSolver(lanes=2048, tt_log2=10) as eager:
+        for rep in range(4):
+            i = rep % 3
+            out, st = g.solve(hs[i])
+            assert np.array_equal(out, c1["tables"][i * 500:(i + 1) * 500]), rep
+            assert st.jobs_done == 2500
+            e, _ = eager.solve(hs[(i + 1) % 3])
+            assert np.array_equal(e, c1["tables"][((i + 1) % 3) * 500:((i + 1) % 3 + 1) * 500])
