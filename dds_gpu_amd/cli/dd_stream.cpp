@@ -14,7 +14,12 @@
 //   RING         ring slots (default 4096)
 //   RATE         bench: Poisson arrivals at RATE deals/s (default 0 = as fast as slots free)
 //   SEED         bench: deal stream seed (default 2026)
-// Bench prints the latency from arrival to result: p50 and p99 in ms.
+//   TEAM=16      sixteen lanes per search (THREADS / 16 searches)
+//   MODEL=FILE   a trained trick-count net: each deal's 20 predictions are computed on the GPU
+//                when it arrives (inside its latency) and seed the search (DDX_SEED_CERT)
+// Bench prints the latency from arrival to result, p50 and p99 in ms, over the deals that
+// arrived in the first half of the run. A deal still unsolved at the end counts with the
+// lower bound (end - arrival); a percentile that falls on such a deal is printed as ">=".
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -57,11 +62,21 @@ int main(int argc, char** argv) {
   cfg.lanes = (threads + 255) / 256 * 256;
   cfg.tt_log2 = tt_log2;
   cfg.jobs_per_deal = getenv("PER_LEAD") && !atoi(getenv("PER_LEAD")) ? 5 : 20;
+  cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
+  ddx_mlp* net = nullptr;
+  if (getenv("MODEL")) {
+    if (ddx_mlp_create(0, 1, &net) != DDX_OK || ddx_mlp_load(net, getenv("MODEL")) != DDX_OK) {
+      fprintf(stderr, "%s\n", ddx_last_error());
+      return 1;
+    }
+    cfg.seed_mode = DDX_SEED_CERT;
+  }
   const int ring = getenv("RING") ? atoi(getenv("RING")) : 4096;
   const double rate = getenv("RATE") ? atof(getenv("RATE")) : 0;
   const uint64_t seed = getenv("SEED") ? strtoull(getenv("SEED"), 0, 10) : 2026;
-  fprintf(stderr, "%lld threads, per-lane TT %.2f GB total, %d jobs per deal, ring %d deals\n",
-          (long long)cfg.lanes, 320.0 * cfg.lanes * (1 << tt_log2) / 1e9, cfg.jobs_per_deal, ring);
+  fprintf(stderr, "%lld threads (%d per search), TT %.2f GB total, %d jobs per deal, ring %d deals%s\n",
+          (long long)cfg.lanes, cfg.team, 320.0 * cfg.lanes / cfg.team * (1 << tt_log2) / 1e9,
+          cfg.jobs_per_deal, ring, net ? ", learned seeds" : "");
   ddx_stream* s = nullptr;
   if (ddx_stream_open(&cfg, ring, &s) != DDX_OK) {
     fprintf(stderr, "%s\n", ddx_last_error());
@@ -79,6 +94,7 @@ int main(int argc, char** argv) {
     std::exponential_distribution<double> gap(rate > 0 ? rate : 1.0);
     double next_arrival = 0;
     uint64_t h[4];
+    uint8_t seeds[20];
     for (long long p = 0; producing; p++) {
       if (bench) {
         ddx_random_deals(seed, p, 1, 13, h);
@@ -96,8 +112,12 @@ int main(int argc, char** argv) {
         arrival.push_back(arr);
         if (bench && p % check_every == 0) kept.insert(kept.end(), h, h + 4);
       }
+      if (net && ddx_mlp_predict_host(net, h, 1, seeds) != DDX_OK) {
+        fprintf(stderr, "%s\n", ddx_last_error());
+        break;
+      }
       int rc;
-      while ((rc = ddx_stream_push(s, h, nullptr, p)) == DDX_E_FULL) {
+      while ((rc = ddx_stream_push(s, h, net ? seeds : nullptr, p)) == DDX_E_FULL) {
         if (!producing) break;
         std::this_thread::sleep_for(std::chrono::microseconds(100));
       }
@@ -156,7 +176,7 @@ int main(int argc, char** argv) {
       });
   }
 
-  std::vector<double> lat;
+  std::vector<double> done_at;  // by deal id; < 0 while unsolved
   std::vector<int64_t> ids(4096);
   std::vector<uint8_t> tabs(4096 * 20);
   long long consumed = 0, last_consumed = 0;
@@ -171,12 +191,8 @@ int main(int argc, char** argv) {
     const double t = since(t0);
     for (int i = 0; i < got; i++) {
       const long long id = ids[i];
-      double arr;
-      {
-        std::lock_guard<std::mutex> g(mu);
-        arr = arrival[id];
-      }
-      lat.push_back(t - arr);
+      if ((long long)done_at.size() <= id) done_at.resize(id + 1, -1.0);
+      done_at[id] = t;
       if (!bench) {
         unsigned char rec[28];
         memcpy(rec, &id, 8);
@@ -211,11 +227,28 @@ int main(int argc, char** argv) {
   }
   checking = false;
   for (auto& t : checkers) t.join();
+  // Latency over the deals that arrived in the first half; unsolved ones are censored at `el`.
+  std::vector<std::pair<double, bool>> lat;  // (ms, exact)
+  const double horizon = bench ? el / 2 : el;
+  for (size_t id = 0; id < arrival.size(); id++) {
+    if (arrival[id] > horizon) continue;
+    const bool fin = id < done_at.size() && done_at[id] >= 0;
+    lat.push_back({1000 * ((fin ? done_at[id] : el) - arrival[id]), fin});
+  }
   std::sort(lat.begin(), lat.end());
-  auto pct = [&](double q) { return lat.empty() ? 0.0 : 1000 * lat[(size_t)(q * (lat.size() - 1))]; };
+  auto pct = [&](double q, char* buf) {
+    if (lat.empty()) return (char*)"-";
+    const auto& v = lat[(size_t)(q * (lat.size() - 1))];
+    snprintf(buf, 32, "%s%.0f", v.second ? "" : ">=", v.first);
+    return buf;
+  };
+  long long unsolved = 0;
+  for (auto& v : lat) unsolved += !v.second;
+  char b50[32], b99[32];
   fprintf(stderr, "%lld deals solved in %.1f s (%.1f deals/s), CPU check: %ld wrong of %ld\n", consumed,
           el, consumed / el, wrong.load(), checked.load());
-  fprintf(stderr, "latency ms: p50 %.0f p99 %.0f (arrival to result, %zu deals)\n", pct(0.5), pct(0.99),
-          lat.size());
+  fprintf(stderr, "latency ms: p50 %s p99 %s (arrival to result, %zu deals arrived by %.0f s, %lld unsolved)\n",
+          pct(0.5, b50), pct(0.99, b99), lat.size(), horizon, unsolved);
+  if (net) ddx_mlp_destroy(net);
   return rc ? rc : wrong.load() != 0;
 }

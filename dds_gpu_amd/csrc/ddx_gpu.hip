@@ -46,6 +46,7 @@ struct KernelArgs {
   int seed_mode;
   LaneView lanes;
   unsigned long long* counters;  // [0] next job, [1] nodes, [2] probes, [3] hits, [4] jobs done,
+                                 // [5] finished jobs so far, [8] nodes of the finished jobs so far
                                  // [5] progress (jobs done, updated while running),
                                  // [6] TT entries read, [7] TT stores
   volatile int* stop;            // host-mapped; nonzero = stop
@@ -53,7 +54,7 @@ struct KernelArgs {
 };
 
 template <int SCHED, bool PROF, int TEAM>
-__global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
+__device__ __forceinline__ void solve_body(const KernelArgs& a) {
   using TM = TeamOf<TEAM>;
   using X = typename TM::Ops;
   const long long lane = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / TEAM;  // the search
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
   };
   unsigned rounds = 0;
   u32 reported = 0;  // finished jobs already added to the progress counter
+  u64 reported_nodes = 0;
 
   // Cheap stages of one lane, back to back, until it must enter a node or has no work.
   auto cheap_chain = [&]() {
@@ -107,8 +109,12 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
     // The stop flag changes under the kernel: one lane reads it for the whole team.
     if ((++rounds & 4095) == 0 && TM::bcast((int)*a.stop, 0)) L.stage = ST_EXIT;
     if (L.jobs_done != reported) {  // progress for the host (one atomic per finished job)
-      if (TM::leader()) atomicAdd(a.counters + 5, (unsigned long long)(L.jobs_done - reported));
+      if (TM::leader()) {
+        atomicAdd(a.counters + 5, (unsigned long long)(L.jobs_done - reported));
+        atomicAdd(a.counters + 8, (unsigned long long)(L.c.nodes - reported_nodes));
+      }
       reported = L.jobs_done;
+      reported_nodes = L.c.nodes;
     }
     if (SCHED == DDX_SCHED_PRIORITY) {
       if (__all(L.stage == ST_EXIT)) break;
@@ -170,6 +176,23 @@ __global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
     for (int i = 0; i < EX_COUNT; i++)
       if (p_exit[i]) atomicAdd(a.prof + 16 + i, (unsigned long long)p_exit[i]);
   }
+}
+
+// The solver at the compiler's register allocation (2 waves per SIMD: ~230 VGPRs one lane per
+// search, ~210 for teams) ...
+template <int SCHED, bool PROF, int TEAM>
+__global__ __launch_bounds__(256) void solve_kernel(KernelArgs a) {
+  solve_body<SCHED, PROF, TEAM>(a);
+}
+// ... or held to W waves per SIMD (168 VGPRs for 3, 128 for 4): more waves to hide memory
+// latency, at the price of spilling the coldest values to scratch.
+template <int TEAM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void solve_kernel_w3(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, TEAM>(a);
+}
+template <int TEAM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void solve_kernel_w4(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, TEAM>(a);
 }
 
 namespace ddx {
@@ -279,6 +302,7 @@ extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
   cfg->profile = 0;
   cfg->team = 1;
   cfg->graph = 0;
+  cfg->waves_per_simd = 0;
 }
 
 static void free_all(ddx_gpu* h) {
@@ -305,7 +329,9 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 64 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.schedule < 0 ||
       cfg.schedule > 2 || cfg.seed_mode < 0 || cfg.seed_mode > 3 || cfg.block < 64 ||
-      cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != 16)) {
+      cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != 16) ||
+      (cfg.waves_per_simd != 0 &&
+       (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile))) {
     ddx_set_error("ddx_gpu_open: bad config (lanes %lld must be a multiple of block %d and of 64, "
                   "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20, team 1 or 16)",
                   (long long)cfg.lanes, cfg.block, cfg.tt_log2);
@@ -331,7 +357,7 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
     delete h;
     return rc;
   }
-  if (hipMalloc(&h->counters, 8 * 8) != hipSuccess) return fail("counters");
+  if (hipMalloc(&h->counters, 16 * 8) != hipSuccess) return fail("counters");
   if (hipMalloc(&h->prof, 32 * 8) != hipSuccess) return fail("profile counters");
   if (hipHostMalloc((void**)&h->h_stop, 64, hipHostMallocMapped) != hipSuccess) return fail("stop flag");
   *h->h_stop = 0;
@@ -375,7 +401,7 @@ static void launch_solve(const KernelArgs& a, int grid, int block, int team, hip
 // solver. Kernels only (no memset nodes), so the sequence can be captured in a graph.
 static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t s) {
   int rc;
-  if ((rc = launch_fill(h->counters, 64, 0, s)) != DDX_OK) return rc;
+  if ((rc = launch_fill(h->counters, 128, 0, s)) != DDX_OK) return rc;
   if (n) {
     int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, a.out, (size_t)(20 * n),
@@ -385,6 +411,15 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
   const int block = h->cfg.block;
   const int grid = (int)(h->cfg.lanes / block);
   const bool P = h->cfg.profile != 0;
+  const int W = h->cfg.waves_per_simd;
+  if (W == 3 || W == 4) {  // split schedule, no profile (checked at open)
+    if (W == 3 && h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w3<16>, dim3(grid), dim3(block), 0, s, a);
+    else if (W == 3) hipLaunchKernelGGL(solve_kernel_w3<1>, dim3(grid), dim3(block), 0, s, a);
+    else if (h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w4<16>, dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL(solve_kernel_w4<1>, dim3(grid), dim3(block), 0, s, a);
+    HIPCHECK(hipGetLastError());
+    return DDX_OK;
+  }
   switch (h->cfg.schedule * 2 + (P ? 1 : 0)) {
     case 0: launch_solve<DDX_SCHED_PRIORITY, false>(a, grid, block, h->cfg.team, s); break;
     case 1: launch_solve<DDX_SCHED_PRIORITY, true>(a, grid, block, h->cfg.team, s); break;
@@ -532,6 +567,17 @@ extern "C" int ddx_gpu_progress(ddx_gpu* h, uint64_t* jobs_done) {
   HIPCHECK(hipMemcpyAsync(&v, h->counters + 5, 8, hipMemcpyDeviceToHost, h->side));
   HIPCHECK(hipStreamSynchronize(h->side));
   *jobs_done = v;
+  return DDX_OK;
+}
+
+extern "C" int ddx_gpu_progress_nodes(ddx_gpu* h, uint64_t* jobs_done, uint64_t* nodes) {
+  if (!h || !jobs_done || !nodes) return DDX_E_ARG;
+  HIPCHECK(hipSetDevice(h->cfg.device));
+  unsigned long long v[16];
+  HIPCHECK(hipMemcpyAsync(v, h->counters, sizeof v, hipMemcpyDeviceToHost, h->side));
+  HIPCHECK(hipStreamSynchronize(h->side));
+  *jobs_done = v[5];
+  *nodes = v[8];
   return DDX_OK;
 }
 

@@ -84,8 +84,8 @@ __global__ void encode_kernel(const uint64_t* hands, const uint8_t* tables, long
     row[w] = v;
   }
   if (labels) {
-    const int t = tables[s];
-    labels[s] = (decl & 1) == 0 ? t : 13 - t;
+    const int t = tables[s];  // 255: unknown (unsolved), masked out of the loss
+    labels[s] = t == 255 ? -1 : (decl & 1) == 0 ? t : 13 - t;
   }
 }
 
@@ -139,7 +139,8 @@ __global__ void bias_relu_kernel(const float* Z, const float* bias, bf16* Y, lon
 }
 
 // Softmax cross-entropy over the 14 used logits (+ bias); dZ = (p - y) / M as bf16; sums the
-// loss and the count of right argmax predictions into stats[0], stats[1].
+// loss and the count of right argmax predictions into stats[0], stats[1]. Rows with label -1
+// (unknown) add no loss and get a zero gradient.
 __global__ void softmax_ce_kernel(const float* Z, const float* b3, const int* labels, long long M,
                                   bf16* dZ, float* stats) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,9 +159,11 @@ __global__ void softmax_ce_kernel(const float* Z, const float* b3, const int* la
     float sum = 0.f;
     for (int j = 0; j < OUT_USED; j++) sum += expf(z[j] - mx);
     const int y = labels[s];
-    loss = logf(sum) - (z[y] - mx);
-    right = arg == y ? 1.f : 0.f;
-    const float inv = 1.f / (float)M;
+    if (y >= 0) {
+      loss = logf(sum) - (z[y] - mx);
+      right = arg == y ? 1.f : 0.f;
+    }
+    const float inv = y >= 0 ? 1.f / (float)M : 0.f;
     for (int j = 0; j < OUT; j++) {
       const float p = j < OUT_USED ? expf(z[j] - mx) / sum : 0.f;
       dZ[s * OUT + j] = to_bf16(((j == y ? p - 1.f : p)) * inv);

@@ -32,7 +32,7 @@ class GpuConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("lanes", ctypes.c_int64), ("tt_log2", ctypes.c_int),
                 ("jobs_per_deal", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("seed_mode", ctypes.c_int), ("block", ctypes.c_int), ("profile", ctypes.c_int),
-                ("team", ctypes.c_int), ("graph", ctypes.c_int)]
+                ("team", ctypes.c_int), ("graph", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
 
 
 class DdxError(RuntimeError):
@@ -190,12 +190,14 @@ class GpuSolver:
     """A set of solver lanes on one GPU (ddx_gpu_open / ddx_gpu_solve)."""
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=5, schedule=SCHED_CHAIN,
-                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1, graph=False):
+                 seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1, graph=False,
+                 waves_per_simd=0):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
         cfg.jobs_per_deal, cfg.schedule, cfg.seed_mode = jobs_per_deal, schedule, seed_mode
         cfg.block, cfg.profile, cfg.team, cfg.graph = block, int(profile), team, int(graph)
+        cfg.waves_per_simd = waves_per_simd
         self.cfg = cfg
         self.h = ctypes.c_void_p()
         _check(lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))
@@ -236,6 +238,12 @@ class GpuSolver:
         v = ctypes.c_uint64()
         _check(lib().ddx_gpu_progress(self.h, ctypes.byref(v)))
         return v.value
+
+    def progress_nodes(self):
+        """(jobs finished, nodes spent up to each search's last finished job) of a running solve."""
+        v, w = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().ddx_gpu_progress_nodes(self.h, ctypes.byref(v), ctypes.byref(w)))
+        return v.value, w.value
 
     def profile(self, reset=True):
         v = np.zeros(32, np.uint64)

@@ -121,6 +121,9 @@ typedef struct ddx_gpu_config {
                            TT, an order of magnitude faster per search) */
   int graph;            /* 1: launch each batch as a captured HIP graph (clear + solve kernels),
                            re-captured only when the buffers, size or stream change */
+  int waves_per_simd;   /* 0: the compiler's register allocation (2 waves per SIMD); 3 or 4: a
+                           kernel held to that many waves per SIMD (fewer VGPRs, some spilled to
+                           scratch); split schedule only, no profile */
 } ddx_gpu_config;
 
 enum {
@@ -156,6 +159,10 @@ int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats);
 void ddx_gpu_request_stop(ddx_gpu* h);
 /* Jobs finished so far by the running or last solve (callable while it runs). */
 int ddx_gpu_progress(ddx_gpu* h, uint64_t* jobs_done);
+/* The same, plus the nodes its searches spent up to their last finished job (for a live node
+ * rate over a window of a running solve). No counterpart in the reference, which reports
+ * nodes only per finished run. */
+int ddx_gpu_progress_nodes(ddx_gpu* h, uint64_t* jobs_done, uint64_t* nodes);
 /* Duration in ms of the last solve kernel, from HIP events on its stream. */
 double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
 /* Per-stage counters of the last solves (config.profile = 1), per wave64: out32[0..7] rounds by

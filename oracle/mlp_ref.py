@@ -34,7 +34,8 @@ def bf16(x):
 
 
 def encode(hands, tables):
-    """X (20n, 224) one-hot features and labels (NS tricks) for deals hands (n, 4)."""
+    """X (20n, 224) one-hot features and labels (NS tricks) for deals hands (n, 4); a table
+    byte of 255 (unsolved) gives label -1, which the loss skips."""
     n = len(hands)
     X = np.zeros((n * 20, IN), np.float32)
     y = np.zeros(n * 20, np.int64)
@@ -50,7 +51,7 @@ def encode(hands, tables):
                 X[s, 208 + strain] = 1.0
                 X[s, 213 + ((decl + 1) & 3)] = 1.0
                 t = int(tables[d, strain * 4 + decl])
-                y[s] = t if decl % 2 == 0 else 13 - t
+                y[s] = -1 if t == 255 else t if decl % 2 == 0 else 13 - t
     return X, y
 
 
@@ -67,11 +68,14 @@ def loss_and_grads(p, X, y, rounding=True, dtype=np.float32):
     Z = Z - Z.max(axis=1, keepdims=True)
     E = np.exp(Z)
     P = E / E.sum(axis=1, keepdims=True)
-    loss = float(np.mean(-np.log(P[np.arange(M), y])))
-    acc = float(np.mean(P.argmax(axis=1) == y))
+    known = y >= 0
+    yk = np.where(known, y, 0)
+    loss = float(np.sum(np.where(known, -np.log(P[np.arange(M), yk]), 0.0)) / M)
+    acc = float(np.sum(known & (P.argmax(axis=1) == y)) / M)
     dZ = np.zeros((M, OUT), dtype)
     dZ[:, :OUT_USED] = P
-    dZ[np.arange(M), y] -= 1
+    dZ[np.arange(M), yk] -= 1
+    dZ[~known] = 0
     dZ = r(dZ / M).astype(dtype)
     g = {"W3": dZ.T @ H2, "b3": dZ.sum(0)}
     dH2 = r((dZ @ W3) * (H2 != 0)).astype(dtype)

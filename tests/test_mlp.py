@@ -59,17 +59,39 @@ def test_oracle_encoding(batch):
     assert (X[:, 217:] == 0).all() and y.min() >= 0 and y.max() <= 13
 
 
+def test_oracle_unknown_labels_are_skipped(batch):
+    """Unsolved table bytes (255) drop out: the loss and gradients with masked rows equal those
+    of the known rows alone, rescaled by known / all (the mean runs over every row)."""
+    hands, tables = batch
+    t = tables.copy()
+    t[:, ::3] = 255
+    X, y = mlp_ref.encode(hands, t)
+    assert (y.reshape(-1, 20)[:, ::3] == -1).all()
+    p = random_params(6).astype(np.float64)
+    l_all, _, g_all = mlp_ref.loss_and_grads(p, X, y, rounding=False, dtype=np.float64)
+    k = y >= 0
+    l_k, _, g_k = mlp_ref.loss_and_grads(p, X[k], y[k], rounding=False, dtype=np.float64)
+    f = k.sum() / len(y)
+    assert abs(l_all - f * l_k) < 1e-12
+    assert np.allclose(g_all, f * g_k, rtol=1e-10, atol=1e-14)
+
+
 def test_oracle_bf16_rounding():
     x = np.array([1.0, 1.00390625, 1.005859375, -2.5, 3.1415926], np.float32)
     assert list(mlp_ref.bf16(x)) == [1.0, 1.0, 1.0078125, -2.5, 3.140625]
 
 
 @pytest.mark.gpu
-def test_gpu_gradients_match_oracle(ddx, batch):
+@pytest.mark.parametrize("masked", [False, True])
+def test_gpu_gradients_match_oracle(ddx, batch, masked):
     """Loss and all 127,504 gradient entries of the GPU kernels vs the oracle with the same
     BF16 rounding points; tolerance covers FP32 summation order (and its rare one-ulp BF16
-    flips): relative L2 error below 2e-2 per tensor."""
+    flips): relative L2 error below 2e-2 per tensor. masked: a third of the table bytes are
+    255 (unsolved) and must drop out of the loss."""
     hands, tables = batch
+    if masked:
+        tables = tables.copy()
+        tables[:, ::3] = 255
     net = ddx.TrickNet(seed=3)
     p = random_params(4)
     net.set_params(p)
