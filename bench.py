@@ -50,7 +50,7 @@ sys.path.insert(0, ROOT)
 METRIC = "full-deal DD tables/sec on 1 MI355X, learned-seeded vs plain, bit-exact; p99 ms"
 STREAM_SEED = 2026
 HBM_PEAK_GBS = 8000.0
-PLAIN = {"lanes": 65536, "team": 1, "tt_log2": 10, "jobs_per_deal": 5, "schedule": 2}
+PLAIN = {"lanes": 65536, "team": 1, "tt_log2": 10, "jobs_per_deal": 5, "schedule": 2, "waves_per_simd": 0}
 DEFAULT_MODEL = os.path.join(ROOT, "dds_gpu_amd", "models", "tricknet.bin")
 DEFAULT_TUNED = os.path.join(ROOT, "dds_gpu_amd", "tuned.json")
 
@@ -71,7 +71,28 @@ def parse():
     ap.add_argument("--tuned", default=DEFAULT_TUNED)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="deals for the CPU baseline (-1: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 5 plain 1 GiB device-to-device copies (a known-size kernel "
+                         "against which HBM byte counters are scaled)")
     return ap.parse_args()
+
+
+def copy_calibration(torch):
+    """Five 1 GiB device copies (torch's elementwise copy kernel): 1 GiB read + 1 GiB written
+    per dispatch. Under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE they give the counters' scale."""
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    log("copy calibration: 1 GiB copy %.3f ms = %.0f GB/s (read + write)" % (ms, 2 * (1 << 30) / ms / 1e6))
+    del x, y
+    return {"bytes_read": 1 << 30, "bytes_written": 1 << 30, "ms": round(ms, 4)}
 
 
 def cpu_baseline(ddx, hands, tables, sample):
@@ -110,7 +131,7 @@ def run_leg(ddx, torch, name, cfg, d_hands, n, a, gold, seeds=None, predict_s=0.
     d_out = torch.empty((n, 20), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     g = ddx.GpuSolver(lanes=cfg["lanes"], tt_log2=cfg["tt_log2"], jobs_per_deal=cfg["jobs_per_deal"],
-                      schedule=cfg["schedule"], team=cfg["team"],
+                      schedule=cfg["schedule"], team=cfg["team"], waves_per_simd=cfg["waves_per_simd"],
                       seed_mode=ddx.SEED_CERT if seeds is not None else ddx.SEED_PLAIN)
     jpd = cfg["jobs_per_deal"]
     log("[%s] %s, %.1f GB of lane state" % (name, json.dumps(cfg), g.device_bytes / 1e9))
@@ -180,12 +201,13 @@ def main():
     tuned = dict(PLAIN)
     if os.path.exists(a.tuned):
         b = json.load(open(a.tuned))["best"]
-        tuned = {k: b[k] for k in PLAIN}
+        tuned = {k: b.get(k, PLAIN[k]) for k in PLAIN}
     legs = a.legs.split(",")
     have_model = os.path.exists(a.model)
     if "learned" in legs and not have_model:
         log("no trained net at %s: learned leg skipped" % a.model)
         legs.remove("learned")
+    calib = copy_calibration(torch) if a.copy_calibration else None
     results = {}
     for name in legs:
         if name == "plain":
@@ -247,6 +269,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if calib:
+        res["copy_calibration"] = calib
     print(json.dumps(res), flush=True)
     if wrong or (cpu and not cpu["verified"]):
         sys.exit(1)

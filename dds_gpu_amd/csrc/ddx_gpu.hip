@@ -50,8 +50,28 @@ struct KernelArgs {
                                  // [5] progress (jobs done, updated while running),
                                  // [6] TT entries read, [7] TT stores
   volatile int* stop;            // host-mapped; nonzero = stop
+  unsigned long long* wave_prog;  // host-mapped, 2 per wave: finished jobs, nodes up to them
   unsigned long long* prof;      // rounds [0..7], lanes [8..15], exits [16..22], cycles [24..31]
 };
+
+// Publishes the wave's finished jobs and the nodes its searches spent up to them to its two
+// slots of host memory (plain stores by one lane), so the host reads progress without any GPU
+// work -- a copy would have to wait for a free CU while the persistent kernel holds them all.
+// Called with the whole wave converged; `leader` marks the lanes that own a search's counters.
+__device__ __forceinline__ void publish_wave_progress(unsigned long long* wave_prog, bool leader, u32 jobs,
+                                                      u64 nodes) {
+  unsigned long long j = leader ? jobs : 0, n = leader ? nodes : 0;
+#pragma unroll
+  for (int m = 32; m; m >>= 1) {
+    j += __shfl_xor(j, m);
+    n += __shfl_xor(n, m);
+  }
+  if (__lane_id() == 0) {
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    wave_prog[2 * w] = j;
+    wave_prog[2 * w + 1] = n;
+  }
+}
 
 template <int SCHED, bool PROF, int TEAM>
 __device__ __forceinline__ void solve_body(const KernelArgs& a) {
@@ -108,6 +128,7 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   for (;;) {
     // The stop flag changes under the kernel: one lane reads it for the whole team.
     if ((++rounds & 4095) == 0 && TM::bcast((int)*a.stop, 0)) L.stage = ST_EXIT;
+    if ((rounds & 255) == 0) publish_wave_progress(a.wave_prog, TM::leader(), L.jobs_done, reported_nodes);
     if (L.jobs_done != reported) {  // progress for the host (one atomic per finished job)
       if (TM::leader()) {
         atomicAdd(a.counters + 5, (unsigned long long)(L.jobs_done - reported));
@@ -157,6 +178,7 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
       prof_round(ST_ENTER, run, t0);
     }
   }
+  publish_wave_progress(a.wave_prog, TM::leader(), L.jobs_done, reported_nodes);
   if (!TM::leader()) return;
   a.lanes.gen[lane] = L.c.gen;
   atomicAdd(a.counters + 1, (unsigned long long)L.c.nodes);
@@ -270,6 +292,8 @@ struct ddx_gpu {
   unsigned long long* prof = nullptr;
   int* h_stop = nullptr;  // host-mapped
   int* d_stop = nullptr;
+  unsigned long long* h_prog = nullptr;  // host-mapped wave progress slots (2 per wave)
+  unsigned long long* d_prog = nullptr;
   // staging for ddx_gpu_solve (host buffers in, host buffers out)
   u64* d_hands = nullptr;
   uint8_t* d_out = nullptr;
@@ -310,6 +334,7 @@ static void free_all(ddx_gpu* h) {
   if (h->counters) (void)hipFree(h->counters);
   if (h->prof) (void)hipFree(h->prof);
   if (h->h_stop) (void)hipHostFree(h->h_stop);
+  if (h->h_prog) (void)hipHostFree(h->h_prog);
   if (h->d_hands) (void)hipFree(h->d_hands);
   if (h->d_out) (void)hipFree(h->d_out);
   if (h->d_seeds) (void)hipFree(h->d_seeds);
@@ -362,6 +387,10 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   if (hipHostMalloc((void**)&h->h_stop, 64, hipHostMallocMapped) != hipSuccess) return fail("stop flag");
   *h->h_stop = 0;
   if (hipHostGetDevicePointer((void**)&h->d_stop, h->h_stop, 0) != hipSuccess) return fail("stop flag");
+  const size_t prog_bytes = (size_t)(cfg.lanes / 64) * 16;
+  if (hipHostMalloc((void**)&h->h_prog, prog_bytes, hipHostMallocMapped) != hipSuccess) return fail("progress slots");
+  memset(h->h_prog, 0, prog_bytes);
+  if (hipHostGetDevicePointer((void**)&h->d_prog, h->h_prog, 0) != hipSuccess) return fail("progress slots");
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return fail("events");
   if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess ||
@@ -402,6 +431,7 @@ static void launch_solve(const KernelArgs& a, int grid, int block, int team, hip
 static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t s) {
   int rc;
   if ((rc = launch_fill(h->counters, 128, 0, s)) != DDX_OK) return rc;
+  if ((rc = launch_fill(h->d_prog, (size_t)(h->cfg.lanes / 64) * 16, 0, s)) != DDX_OK) return rc;
   if (n) {
     int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, a.out, (size_t)(20 * n),
@@ -455,6 +485,7 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
   a.lanes = lanes_view(h->lanes);
   a.counters = h->counters;
   a.stop = h->d_stop;
+  a.wave_prog = h->d_prog;
   a.prof = h->prof;
   h->t0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(h->ev0, s));
@@ -563,21 +594,22 @@ extern "C" int ddx_gpu_solve(ddx_gpu* h, const uint64_t* hands, int64_t n, const
 extern "C" int ddx_gpu_progress(ddx_gpu* h, uint64_t* jobs_done) {
   if (!h || !jobs_done) return DDX_E_ARG;
   HIPCHECK(hipSetDevice(h->cfg.device));
-  unsigned long long v = 0;
-  HIPCHECK(hipMemcpyAsync(&v, h->counters + 5, 8, hipMemcpyDeviceToHost, h->side));
-  HIPCHECK(hipStreamSynchronize(h->side));
-  *jobs_done = v;
-  return DDX_OK;
+  uint64_t nodes;
+  return ddx_gpu_progress_nodes(h, jobs_done, &nodes);
 }
 
 extern "C" int ddx_gpu_progress_nodes(ddx_gpu* h, uint64_t* jobs_done, uint64_t* nodes) {
   if (!h || !jobs_done || !nodes) return DDX_E_ARG;
-  HIPCHECK(hipSetDevice(h->cfg.device));
-  unsigned long long v[16];
-  HIPCHECK(hipMemcpyAsync(v, h->counters, sizeof v, hipMemcpyDeviceToHost, h->side));
-  HIPCHECK(hipStreamSynchronize(h->side));
-  *jobs_done = v[5];
-  *nodes = v[8];
+  // Host memory written by the kernel's waves: no GPU call, so it answers while the kernel
+  // occupies every CU. Each slot is one aligned 8-byte store, so reads never tear.
+  const volatile unsigned long long* p = h->h_prog;
+  uint64_t j = 0, n = 0;
+  for (int64_t w = 0; w < h->cfg.lanes / 64; w++) {
+    j += p[2 * w];
+    n += p[2 * w + 1];
+  }
+  *jobs_done = j;
+  *nodes = n;
   return DDX_OK;
 }
 

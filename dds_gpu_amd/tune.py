@@ -15,7 +15,8 @@ candidate sees the same queue order. Every answer a candidate finishes is compar
 first candidate's answers for the same deals (all candidates are exact solvers; a difference
 is a bug and aborts the sweep).
 
-Swept: lanes (threads per CU x 256 CUs), lanes per search (one lane, or a 16-lane team), TT_LOG2
+Swept: lanes (threads per CU x 256 CUs, with the kernel held to 3 or 4 waves per SIMD where
+the threads need it), lanes per search (one lane, or a 16-lane team), TT_LOG2
 per search (one lane: 10..12, i.e. 0.3-1.3 MB of TT per search and up to 86 GB in all; a team:
 13..14), jobs per deal (5 = one job per strain, 20 = one per lead) and the stage schedule
 (priority / chain / split). The best is written to `out` under the workload's name; bench.py
@@ -52,16 +53,19 @@ def log(*a):
 
 
 def candidates(quick):
+    """(threads per CU, waves per SIMD it needs) x team x TT_LOG2 x schedule x jobs per deal.
+    256 threads per CU is one wave per SIMD; 512 two (the compiler's allocation); 768 and
+    1024 need the kernels held to 3 or 4 waves per SIMD."""
     out = []
-    for team, tpc in [(1, 256), (1, 512), (16, 256), (16, 512)]:
+    for team, tpc, w in [(1, 256, 0), (1, 512, 0), (1, 768, 3), (1, 1024, 4), (16, 512, 0), (16, 768, 3)]:
         tts = (10, 11, 12) if team == 1 else (13, 14)
         for tt, sched, jpd in itertools.product(tts, (0, 1, 2), (5, 20)):
-            if quick and sched != 2:
+            if (quick or w) and sched != 2:
                 continue
-            if team == 1 and tpc == 512 and tt == 12:
-                continue  # 172 GB of TT: fits, but leaves little for anything else
+            if team == 1 and tpc * CUS * 320 * (1 << tt) > 180e9:
+                continue  # keep the TTs within ~180 GB of the 288 GB
             out.append({"lanes": CUS * tpc, "team": team, "tt_log2": tt, "schedule": sched,
-                        "jobs_per_deal": jpd})
+                        "jobs_per_deal": jpd, "waves_per_simd": w})
     return out
 
 

@@ -17,13 +17,40 @@
 // return the exact trick count; seeds only change how many searches it takes.
 //
 // Stages: JOB (take a job), ROOT (start a lead), ENTER (enter a node), NEXT (play the next
-// move of the top frame), RET (a child returned), EXIT.
+// move of the top frame), RET (a child returned), EXIT, and UNIT (a helper subproblem --
+// a position with a trick window, see dds_gpu_amd/csrc/ddx_help.hpp -- has its value in lo).
 #pragma once
 #include "search.hpp"
 
 namespace ddx {
 
-enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_COUNT };
+enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_UNIT, ST_COUNT };
+
+// First-writer-wins flag and counter for answers that helpers may also produce (device
+// atomics; a host build is single-threaded per flag).
+DDX_FN bool claim_flag(u32* f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicCAS(f, 0u, 1u) == 0u;
+#else
+  if (*f) return false;
+  *f = 1;
+  return true;
+#endif
+}
+DDX_FN void count_one(unsigned long long* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd(c, 1ull);
+#else
+  ++*c;
+#endif
+}
+DDX_FN bool flag_set(const u32* f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+#else
+  return *f != 0;
+#endif
+}
 enum { SEED_PLAIN = 0, SEED_STEP = 1, SEED_CERT = 2, SEED_CERTP = 3 };
 
 // The batch a lane works on. Deal d's hands are hands[4d..4d+3]; its results go to
@@ -36,6 +63,10 @@ struct Batch {
   long long jobs;        // deals * jobs_per_deal
   int jobs_per_deal;     // 5 (per strain) or 20 (per lead)
   int seed_mode;
+  // With helpers: one flag per answer (same layout as out); the first to finish a lead -- its
+  // own lane or the helpers -- claims it, writes out and counts it in *leads_done.
+  u32* lead_done = nullptr;
+  unsigned long long* leads_done = nullptr;
 };
 
 struct Lane {
@@ -51,6 +82,8 @@ struct Lane {
   int strain, decl, last_decl, prev_ns, last_true;
   int exit_at;  // where the last enter() decided (profiling)
   u32 jobs_done;
+  int rec;     // helper subproblem being solved, or -1 for the batch's own leads
+  u32 serial;  // null-window searches started (identifies the current search)
 };
 
 DDX_FN void lane_init(Lane& L, TTView tt) {
@@ -68,6 +101,8 @@ DDX_FN void lane_init(Lane& L, TTView tt) {
   L.target = L.lo = L.hi = L.searches = 0;
   L.exit_at = 0;
   L.jobs_done = 0;
+  L.rec = -1;
+  L.serial = 0;
 }
 
 DDX_FN void start_search(Lane& L) {
@@ -77,6 +112,7 @@ DDX_FN void start_search(Lane& L) {
   L.p.left = popc(L.p.hand[0]);
   L.p.trick = 0;
   L.depth = 0;
+  L.serial++;
   L.stage = ST_ENTER;
 }
 
@@ -98,8 +134,24 @@ DDX_FN void lane_load_job(Lane& L, const Batch& B, long long job) {
   L.stage = ST_ROOT;
 }
 
-// ROOT: first target of this lead's search sequence.
+// Moves on to the job's next lead, or to the next job.
+DDX_FN void lane_next_lead(Lane& L) {
+  L.decl++;
+  if (L.decl <= L.last_decl) {
+    L.stage = ST_ROOT;
+    return;
+  }
+  L.jobs_done++;
+  L.stage = ST_JOB;
+}
+
+// ROOT: first target of this lead's search sequence (a lead that helpers already answered is
+// skipped).
 DDX_FN void lane_root(Lane& L, const Batch& B) {
+  if (B.lead_done && flag_set(B.lead_done + 20 * L.deal + 4 * L.strain + L.decl)) {
+    lane_next_lead(L);
+    return;
+  }
   const int left = popc(L.p.hand[0]);
   L.lo = 0;
   L.searches = 0;
@@ -125,22 +177,31 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
   if (L.v) L.lo = L.target;
   else L.hi = L.target - 1;
   if (L.lo < L.hi) {
-    const bool adjacent = B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2);
+    const bool adjacent =
+        L.rec < 0 && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2));
     if (adjacent) L.target = L.v ? L.lo + 1 : L.hi;
     else L.target = (L.lo + L.hi + 1) / 2;
-    start_search(L);
+    if (L.rec >= 0) {  // a subproblem: the search unwound to its root position; search again
+      L.depth = 0;
+      L.serial++;
+      L.stage = ST_ENTER;
+    } else {
+      start_search(L);
+    }
+    return;
+  }
+  if (L.rec >= 0) {
+    L.stage = ST_UNIT;
     return;
   }
   const int ns = L.lo;
-  B.out[20 * L.deal + 4 * L.strain + L.decl] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
-  L.decl++;
-  if (L.decl <= L.last_decl) {
-    L.prev_ns = ns < 1 ? 1 : ns;
-    L.stage = ST_ROOT;
-    return;
+  const long long idx = 20 * L.deal + 4 * L.strain + L.decl;
+  if (!B.lead_done || claim_flag(B.lead_done + idx)) {
+    B.out[idx] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
+    if (B.leads_done) count_one(B.leads_done);
   }
-  L.jobs_done++;
-  L.stage = ST_JOB;
+  L.prev_ns = ns < 1 ? 1 : ns;
+  lane_next_lead(L);
 }
 
 // ENTER: enter the node at the current position; `st` is the lane's frame stack

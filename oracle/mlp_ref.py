@@ -86,3 +86,22 @@ def loss_and_grads(p, X, y, rounding=True, dtype=np.float32):
     g["W3"][OUT_USED:] = 0
     g["b3"][OUT_USED:] = 0
     return loss, acc, pack(g)
+
+
+def load(path):
+    """Parameters from a file written by ddx_mlp_save: "DDXMLP1\\0", int64 count, fp32 values."""
+    with open(path, "rb") as f:
+        magic = f.read(8)
+        cnt = int(np.frombuffer(f.read(8), np.int64)[0])
+        p = np.frombuffer(f.read(4 * cnt), np.float32).copy()
+    assert magic == b"DDXMLP1\0" and cnt == sum(int(np.prod(s)) for _, s in SIZES), "not a net file"
+    return p
+
+
+def predict(p, X):
+    """Argmax NS-trick predictions (BF16 weights and activations, as the GPU forward)."""
+    w = unpack(p.astype(np.float32))
+    H1 = bf16(np.maximum(X @ bf16(w["W1"]).T + w["b1"], 0))
+    H2 = bf16(np.maximum(H1 @ bf16(w["W2"]).T + w["b2"], 0))
+    Z = (H2 @ bf16(w["W3"]).T + w["b3"])[:, :OUT_USED]
+    return Z.argmax(axis=1), Z
