@@ -28,6 +28,13 @@
 //
 // Everything here runs one lane per search (the per-lane kernel).
 #pragma once
+// The machine's helper hooks must be compiled in: include this header before any other ddx
+// header in the translation unit.
+#if defined(DDX_HELP) && !DDX_HELP
+#error "ddx_help.hpp needs DDX_HELP=1: include it before the other ddx headers"
+#endif
+#undef DDX_HELP
+#define DDX_HELP 1
 #include "ddx/machine.hpp"
 
 namespace ddx {

@@ -22,6 +22,12 @@
 #pragma once
 #include "search.hpp"
 
+// DDX_HELP = 1 compiles the hooks for tail helpers (dds_gpu_amd/csrc/ddx_help.hpp, which sets
+// it); without it they fold away and the machine is exactly the plain one.
+#ifndef DDX_HELP
+#define DDX_HELP 0
+#endif
+
 namespace ddx {
 
 enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_UNIT, ST_COUNT };
@@ -63,10 +69,12 @@ struct Batch {
   long long jobs;        // deals * jobs_per_deal
   int jobs_per_deal;     // 5 (per strain) or 20 (per lead)
   int seed_mode;
+#if DDX_HELP
   // With helpers: one flag per answer (same layout as out); the first to finish a lead -- its
   // own lane or the helpers -- claims it, writes out and counts it in *leads_done.
   u32* lead_done = nullptr;
   unsigned long long* leads_done = nullptr;
+#endif
 };
 
 struct Lane {
@@ -148,10 +156,12 @@ DDX_FN void lane_next_lead(Lane& L) {
 // ROOT: first target of this lead's search sequence (a lead that helpers already answered is
 // skipped).
 DDX_FN void lane_root(Lane& L, const Batch& B) {
+#if DDX_HELP
   if (B.lead_done && flag_set(B.lead_done + 20 * L.deal + 4 * L.strain + L.decl)) {
     lane_next_lead(L);
     return;
   }
+#endif
   const int left = popc(L.p.hand[0]);
   L.lo = 0;
   L.searches = 0;
@@ -177,11 +187,12 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
   if (L.v) L.lo = L.target;
   else L.hi = L.target - 1;
   if (L.lo < L.hi) {
+    const bool sub = DDX_HELP && L.rec >= 0;
     const bool adjacent =
-        L.rec < 0 && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2));
+        !sub && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2));
     if (adjacent) L.target = L.v ? L.lo + 1 : L.hi;
     else L.target = (L.lo + L.hi + 1) / 2;
-    if (L.rec >= 0) {  // a subproblem: the search unwound to its root position; search again
+    if (sub) {  // a subproblem: the search unwound to its root position; search again
       L.depth = 0;
       L.serial++;
       L.stage = ST_ENTER;
@@ -190,18 +201,28 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
     }
     return;
   }
-  if (L.rec >= 0) {
+  if (DDX_HELP && L.rec >= 0) {
     L.stage = ST_UNIT;
     return;
   }
   const int ns = L.lo;
   const long long idx = 20 * L.deal + 4 * L.strain + L.decl;
+#if DDX_HELP
   if (!B.lead_done || claim_flag(B.lead_done + idx)) {
     B.out[idx] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
     if (B.leads_done) count_one(B.leads_done);
   }
-  L.prev_ns = ns < 1 ? 1 : ns;
-  lane_next_lead(L);
+#else
+  B.out[idx] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
+#endif
+  L.decl++;
+  if (L.decl <= L.last_decl) {
+    L.prev_ns = ns < 1 ? 1 : ns;
+    L.stage = ST_ROOT;
+    return;
+  }
+  L.jobs_done++;
+  L.stage = ST_JOB;
 }
 
 // ENTER: enter the node at the current position; `st` is the lane's frame stack
