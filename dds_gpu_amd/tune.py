@@ -30,6 +30,7 @@ import sys
 import time
 
 import numpy as np
+import torch  # before the ddx library: one HIP runtime for both (torch holds the device buffers)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
@@ -73,7 +74,6 @@ def candidates(quick):
 
 def measure(cfg, hands, warmup_s, measure_s):
     """Sustained tables/s of one candidate; returns (score, answers, details)."""
-    import torch
     n = len(hands)
     dev = torch.device("cuda:0")
     d_hands = torch.from_numpy(hands.view(np.int64)).to(dev)
