@@ -8,9 +8,10 @@
 //          (reference dd_ns_tricks / dd_solve_strain, src/dd.h:877-891, 926-939)
 //   STEP   from a given seed, step up or down one trick at a time
 //          (reference dd_ns_tricks_from / dd_solve_strain_guess, src/dd.h:898-924)
-//   CERT   probe the seed g and then g+1 (or g-1): a right seed costs two searches, the
-//          certificate "at least g, not g+1"; after that, binary search on what is left.
-//          Seeds come from the caller (e.g. the trick-count net).
+//   CERT   probe the seed g, then step one trick at a time in the direction the answers
+//          point, at most twice more; a right seed costs two searches (the certificate "at
+//          least g, not g+1"), a seed off by one or two three; after three searches, binary
+//          search on what is left. Seeds come from the caller (e.g. the trick-count net).
 //   CERTP  CERT seeded with the previous lead's answer (7 for the first); the no-net
 //          ablation of CERT.
 // Every policy keeps the invariant answer in [lo, hi] and stops at lo == hi, so all of them
@@ -189,7 +190,7 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
   if (L.lo < L.hi) {
     const bool sub = DDX_HELP && L.rec >= 0;
     const bool adjacent =
-        !sub && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 2));
+        !sub && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 3));
     if (adjacent) L.target = L.v ? L.lo + 1 : L.hi;
     else L.target = (L.lo + L.hi + 1) / 2;
     if (sub) {  // a subproblem: the search unwound to its root position; search again
