@@ -234,18 +234,22 @@ def main():
     head = next(k for k in ("learned", "tuned", "plain") if k in results)
     H = results[head]
     kern = H["kernel"]
-    traffic = None
+    # HBM bytes per launch of this leg's kernel, measured by rocprofv3 --pmc FETCH_SIZE /
+    # WRITE_SIZE over this same command and scaled by a 1 GiB copy's counts (the committed
+    # profile names the command); used only when its configuration is this leg's.
+    traffic, extra = None, {}
     tpath = os.path.join(ROOT, "profiles", "solve_kernel_hbm_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("config") == {k: H["config"][k] for k in PLAIN}:
+        if tr.get("config") == H["config"] and tr.get("deals") == n:
             traffic = tr.get("bytes_per_launch")
+            extra = {k: tr[k] for k in ("l2_hit_rate", "valu_busy", "traffic_source") if k in tr}
     roofline = {"bound": "hbm", "achieved": kern["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(kern["achieved_gbs"] / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "kernel": "solve_kernel", "leg": head,
+                "kernel": "solve_kernel", "leg": head, **extra,
                 "note": "integer tree search: TT probes are dependent random 64 B reads and the "
-                        "stage code diverges across lanes, so the kernel is bound by issue and "
-                        "memory latency far below the HBM peak"}
+                        "stage code diverges across lanes; the kernel is bound by instruction "
+                        "issue (VALU busy) under divergence, far below the HBM peak"}
     cpu = None
     if not a.no_cpu_baseline:
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()

@@ -65,7 +65,12 @@ int main(int argc, char** argv) {
   cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
   ddx_mlp* net = nullptr;
   if (getenv("MODEL")) {
-    if (ddx_mlp_create(0, 1, &net) != DDX_OK || ddx_mlp_load(net, getenv("MODEL")) != DDX_OK) {
+    uint64_t h0[4];
+    uint8_t s0[20];
+    ddx_random_deals(1, 0, 1, 13, h0);
+    // Load, then predict once so the net's buffers exist before the persistent kernel starts.
+    if (ddx_mlp_create(0, 1, &net) != DDX_OK || ddx_mlp_load(net, getenv("MODEL")) != DDX_OK ||
+        ddx_mlp_predict_host(net, h0, 1, s0) != DDX_OK) {
       fprintf(stderr, "%s\n", ddx_last_error());
       return 1;
     }

@@ -14,8 +14,9 @@
 //
 // Teams: with team = 16, sixteen lanes run one search together (ddx_team.hpp): the TT tag
 // line is probed and rewritten in one coalesced 64-byte access and the move weights are
-// computed one candidate per lane, so a search advances ~an order of magnitude faster per
-// node while four searches share a wave. Storage is then per search, not per lane.
+// computed one candidate per lane, so a search advances about five times faster (measured
+// under full load) while four searches share a wave. Storage is then per search, not per
+// lane.
 //
 // Wavefront schedules (one wave64 = 64 lanes in lockstep; stages diverge per lane):
 //   PRIORITY  each round runs one stage for the lanes in it: RET > NEXT > ROOT > JOB > ENTER

@@ -266,6 +266,8 @@ struct ddx_mlp {
   uint64_t* d_hands = nullptr;
   uint8_t* d_tables = nullptr;
   long long host_cap = 0;
+  uint8_t* d_seeds = nullptr;  // predictions for ddx_mlp_predict_host
+  long long seeds_cap = 0;
   hipStream_t s = nullptr;
 };
 
@@ -414,7 +416,7 @@ extern "C" void ddx_mlp_destroy(ddx_mlp* h) {
   (void)hipSetDevice(h->device);
   if (h->s) (void)hipStreamSynchronize(h->s);
   mlp_free_acts(h);
-  void* ptrs[] = {h->p, h->g, h->m, h->v, h->w, h->stats, h->d_hands, h->d_tables};
+  void* ptrs[] = {h->p, h->g, h->m, h->v, h->w, h->stats, h->d_hands, h->d_tables, h->d_seeds};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (h->s) (void)hipStreamDestroy(h->s);
@@ -538,17 +540,26 @@ extern "C" int ddx_mlp_predict(ddx_mlp* h, const uint64_t* d_hands, int64_t n, u
   return DDX_OK;
 }
 
+// Host in, host out. Every copy and kernel runs on the handle's non-blocking stream and
+// buffers are reused (no per-call allocation or free, which could wait for the whole device),
+// so predictions can be made while a persistent solver kernel runs on another stream.
 extern "C" int ddx_mlp_predict_host(ddx_mlp* h, const uint64_t* hands, int64_t n, uint8_t* seeds) {
   if (!h || !hands || !seeds || n < 0) return DDX_E_ARG;
   HIPCHECK(hipSetDevice(h->device));
   int rc;
   if ((rc = mlp_stage_host(h, hands, nullptr, n)) != DDX_OK) return rc;
-  uint8_t* d_seeds;
-  HIPCHECK(hipMalloc(&d_seeds, n * 20 + 16));
-  rc = ddx_mlp_predict(h, h->d_hands, n, d_seeds, nullptr);
-  if (rc == DDX_OK && hipMemcpy(seeds, d_seeds, n * 20, hipMemcpyDeviceToHost) != hipSuccess) rc = DDX_E_HIP;
-  (void)hipFree(d_seeds);
-  return rc;
+  if (n > h->seeds_cap) {
+    if (h->d_seeds) (void)hipFree(h->d_seeds);
+    h->d_seeds = nullptr;
+    h->seeds_cap = 0;
+    HIPCHECK(hipMalloc(&h->d_seeds, n * 20 + 16));
+    h->seeds_cap = n;
+  }
+  rc = ddx_mlp_predict(h, h->d_hands, n, h->d_seeds, nullptr);
+  if (rc != DDX_OK) return rc;
+  HIPCHECK(hipMemcpyAsync(seeds, h->d_seeds, n * 20, hipMemcpyDeviceToHost, h->s));
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return DDX_OK;
 }
 
 // File: "DDXMLP1\0", int64 count, then count fp32 parameters.

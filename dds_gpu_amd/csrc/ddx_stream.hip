@@ -223,7 +223,9 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   if (hipMalloc(&s->counters, 64) != hipSuccess ||
       hipMalloc(&s->deal_done, ((s->ring * 4 + 15) / 16) * 16) != hipSuccess)
     return fail("device counters");
-  if (hipStreamCreate(&s->stream) != hipSuccess) return fail("a HIP stream");
+  // Non-blocking: the persistent kernel must not be waited on by work on the null stream
+  // (other libraries' synchronous copies in the same process would otherwise never return).
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail("a HIP stream");
   if ((rc = launch_fill(s->counters, 64, 0, s->stream)) != DDX_OK ||
       (rc = launch_fill(s->deal_done, ((s->ring * 4 + 15) / 16) * 16, 0, s->stream)) != DDX_OK) {
     stream_free(s);

@@ -118,7 +118,7 @@ typedef struct ddx_gpu_config {
   int profile;          /* 1: collect per-stage counters (slower) */
   int team;             /* lanes per search: 1 (one search per lane, as the reference) or 16
                            (a team of 16 lanes per search: lanes/16 searches, each with its own
-                           TT, an order of magnitude faster per search) */
+                           TT, about five times faster per search under full load) */
   int graph;            /* 1: launch each batch as a captured HIP graph (clear + solve kernels),
                            re-captured only when the buffers, size or stream change */
   int waves_per_simd;   /* 0: the compiler's register allocation (2 waves per SIMD); 3 or 4: a

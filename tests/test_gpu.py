@@ -145,6 +145,39 @@ def test_stream_matches_batch(ddx, c1, team, tt_log2):
     assert st.jobs_done == 20 * n
 
 
+def test_stream_with_net_predictions_while_running(ddx, c1):
+    """Seeds predicted one deal at a time on the host while the persistent stream kernel runs
+    (the net's copies and kernels must not wait for that kernel): tables exact."""
+    model = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dds_gpu_amd",
+                         "models", "tricknet.bin")
+    if not os.path.exists(model):
+        pytest.skip("no trained net")
+    n = 300
+    hands = c1["hands"][:n]
+    net = ddx.TrickNet(path=model)
+    net.predict(hands[:1])  # buffers exist before the stream starts
+    got = {}
+    with ddx.Stream(lanes=4096, tt_log2=10, jobs_per_deal=20, seed_mode=ddx.SEED_CERT, ring_slots=256) as s:
+        i = 0
+        t0 = time.time()
+        while len(got) < n and time.time() - t0 < 60:
+            while i < n:
+                seeds = net.predict(hands[i:i + 1])[0]
+                if not s.push(hands[i], i, seeds):
+                    break
+                i += 1
+            ids, tabs = s.poll()
+            for k, t in zip(ids, tabs):
+                got[int(k)] = t
+            if i == n:
+                s.finish()
+            time.sleep(0.001)
+        s.close()
+    net.close()
+    assert len(got) == n
+    assert np.array_equal(np.stack([got[k] for k in range(n)]), c1["tables"][:n])
+
+
 def test_bad_configs_fail_cleanly(ddx):
     with pytest.raises(ddx.DdxError):
         ddx.GpuSolver(lanes=100)
