@@ -32,8 +32,10 @@
 
 #include <chrono>
 #include <new>
+#include <type_traits>
 
 #include "ddx_lanes.hpp"
+#include "ddx_lds.hpp"
 #include "ddx_team.hpp"
 
 using namespace ddx;
@@ -74,10 +76,11 @@ __device__ __forceinline__ void publish_wave_progress(unsigned long long* wave_p
   }
 }
 
-template <int SCHED, bool PROF, int TEAM>
+template <int SCHED, bool PROF, int TEAM, bool LDS = false>
 __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   using TM = TeamOf<TEAM>;
-  using X = typename TM::Ops;
+  using X = typename std::conditional<LDS, LdsOps, typename TM::Ops>::type;
+  if (LDS) lds_reset(lds_lane(), true);
   const long long lane = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / TEAM;  // the search
   Lane L;
   GpuStack st;
@@ -188,6 +191,7 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   atomicAdd(a.counters + 4, (unsigned long long)L.jobs_done);
   atomicAdd(a.counters + 6, (unsigned long long)L.c.ent_reads);
   atomicAdd(a.counters + 7, (unsigned long long)L.c.stores);
+  if (LDS) atomicAdd(a.counters + 9, (unsigned long long)lds_lane()[4 + 16 * kLdsLines]);
   if (PROF) {
     if ((threadIdx.x & 63) == 0) {  // (a team leader; lane 0 of the wave)
       for (int i = 0; i < 8; i++) {
@@ -216,6 +220,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 template <int TEAM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void solve_kernel_w4(KernelArgs a) {
   solve_body<DDX_SCHED_SPLIT, false, TEAM>(a);
+}
+// One lane per search with the LDS cache of tag lines (ddx_lds.hpp), at the compiler's
+// allocation or held to 3 waves per SIMD.
+__global__ __launch_bounds__(256) void solve_kernel_lds(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, 1, true>(a);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void solve_kernel_lds_w3(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, 1, true>(a);
 }
 
 namespace ddx {
@@ -328,6 +340,7 @@ extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
   cfg->team = 1;
   cfg->graph = 0;
   cfg->waves_per_simd = 0;
+  cfg->lds_lines = 0;
 }
 
 static void free_all(ddx_gpu* h) {
@@ -357,7 +370,9 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
       cfg.schedule > 2 || cfg.seed_mode < 0 || cfg.seed_mode > 3 || cfg.block < 64 ||
       cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != 16) ||
       (cfg.waves_per_simd != 0 &&
-       (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile))) {
+       (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile)) ||
+      (cfg.lds_lines != 0 && (cfg.lds_lines != 3 || cfg.team != 1 || cfg.schedule != DDX_SCHED_SPLIT ||
+                              cfg.profile || cfg.waves_per_simd == 4))) {
     ddx_set_error("ddx_gpu_open: bad config (lanes %lld must be a multiple of block %d and of 64, "
                   "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20, team 1 or 16)",
                   (long long)cfg.lanes, cfg.block, cfg.tt_log2);
@@ -443,6 +458,12 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
   const int grid = (int)(h->cfg.lanes / block);
   const bool P = h->cfg.profile != 0;
   const int W = h->cfg.waves_per_simd;
+  if (h->cfg.lds_lines) {  // one lane per search, split schedule, no profile (checked at open)
+    if (W == 3) hipLaunchKernelGGL(solve_kernel_lds_w3, dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL(solve_kernel_lds, dim3(grid), dim3(block), 0, s, a);
+    HIPCHECK(hipGetLastError());
+    return DDX_OK;
+  }
   if (W == 3 || W == 4) {  // split schedule, no profile (checked at open)
     if (W == 3 && h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w3<16>, dim3(grid), dim3(block), 0, s, a);
     else if (W == 3) hipLaunchKernelGGL(solve_kernel_w3<1>, dim3(grid), dim3(block), 0, s, a);
@@ -536,6 +557,7 @@ extern "C" int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats) {
     stats->jobs_done = c[4];
     stats->tt_entry_reads = c[6];
     stats->tt_stores = c[7];
+    stats->tt_line_cache_hits = c[9];
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h->t0).count();
   }
   if ((int64_t)c[4] < h->last_jobs) {

@@ -329,6 +329,7 @@ extern "C" int ddx_stream_finish(ddx_stream* s) {
 
 extern "C" int ddx_stream_close(ddx_stream* s, ddx_stats* stats) {
   if (!s) return DDX_E_ARG;
+  if (stats) memset(stats, 0, sizeof *stats);
   (void)hipSetDevice(s->cfg.device);
   int rc = DDX_OK;
   if (s->launched) {

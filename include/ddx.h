@@ -48,6 +48,7 @@ typedef struct ddx_stats {
   double seconds;        /* wall time of the solve, device work included */
   uint64_t tt_entry_reads; /* 16-byte TT entries read by probes */
   uint64_t tt_stores;    /* TT stores (each rewrites a 64-byte tag line and one entry) */
+  uint64_t tt_line_cache_hits; /* tag lines served by the LDS line cache (config.lds_lines) */
 } ddx_stats;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -124,6 +125,8 @@ typedef struct ddx_gpu_config {
   int waves_per_simd;   /* 0: the compiler's register allocation (2 waves per SIMD); 3 or 4: a
                            kernel held to that many waves per SIMD (fewer VGPRs, some spilled to
                            scratch); split schedule only, no profile */
+  int lds_lines;        /* 0, or 3: each lane keeps its last 3 TT tag lines in LDS (write-through;
+                           one lane per search, split schedule, no profile, waves_per_simd 0/3) */
 } ddx_gpu_config;
 
 enum {

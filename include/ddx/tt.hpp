@@ -135,6 +135,13 @@ DDX_FN void store_line(const TTView& tt, u32 b, const TagLine& L) {
   }
 }
 
+// Where tag lines come from: straight from the TT in memory (the default), or through a
+// per-search cache (the GPU kernel's LDS line cache) with the same interface.
+struct GlobalLines {
+  DDX_M void load(const TTView& tt, u32 b, TagLine& L) const { load_line(tt, b, L); }
+  DDX_M void store(const TTView& tt, u32 b, const TagLine& L) const { store_line(tt, b, L); }
+};
+
 // Mask of tag positions whose tag equals k.tag apart from the way bits.
 DDX_FN u32 match_mask(const TagLine& L, u32 tag) {
   u32 m = 0;
@@ -153,9 +160,11 @@ DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, by masks (
 // cards of the entry), -1 = unknown; then `hint` may get a stored best card. Matching tags
 // are visited newest first; a later matching entry's best card replaces an earlier one
 // (reference dd_tt_probe, src/dd.h:458-493).
-DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint, u64& ent_reads) {
+template <class Lines = GlobalLines>
+DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint, u64& ent_reads,
+                    const Lines& lines = Lines()) {
   TagLine L;
-  load_line(tt, k.bucket, L);
+  lines.load(tt, k.bucket, L);
   u32 m = match_mask(L, k.tag);
   const Entry* base = tt.ents + (size_t)k.bucket * kWays;
   while (m) {
@@ -213,11 +222,13 @@ DDX_FN int take_way(TagLine& L, u32 tag, u32 gen) {
 // tightened in place (its best card kept when none is given); otherwise a way is taken.
 // Classes with more than kMaxRel relevant cards are not stored (reference dd_tt_store,
 // src/dd.h:500-532).
-DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, int best, u32 gen) {
+template <class Lines = GlobalLines>
+DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, int best, u32 gen,
+                     const Lines& lines = Lines()) {
   u64 w1, hi;
   if (!pack_owners(k, rel, w1, hi)) return;
   TagLine L;
-  load_line(tt, k.bucket, L);
+  lines.load(tt, k.bucket, L);
   Entry* base = tt.ents + (size_t)k.bucket * kWays;
   u32 m = match_mask(L, k.tag);
   while (m) {
@@ -236,7 +247,7 @@ DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, in
     }
   }
   int way = take_way(L, k.tag, gen);
-  store_line(tt, k.bucket, L);
+  lines.store(tt, k.bucket, L);
   Entry* e = base + way;
   Entry v;
   v.w0 = pack_w0(k.q, lb, ub, best, hi);

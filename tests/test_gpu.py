@@ -145,6 +145,22 @@ def test_stream_matches_batch(ddx, c1, team, tt_log2):
     assert st.jobs_done == 20 * n
 
 
+@pytest.mark.parametrize("waves,lds", [(3, 0), (4, 0), (0, 3), (3, 3)])
+def test_occupancy_and_lds_variants_exact(ddx, c1, waves, lds):
+    """Kernels held to 3 or 4 waves per SIMD, and the LDS tag-line cache: C1 tables exact; the
+    cache serves a real share of the tag lines."""
+    hands = c1["hands"]
+    with ddx.GpuSolver(lanes=8192, tt_log2=10, jobs_per_deal=5, schedule=2, waves_per_simd=waves,
+                       lds_lines=lds) as g:
+        out, st = g.solve(hands)
+    log("waves %d lds %d: nodes %d, line cache hits %d" % (waves, lds, st.nodes, st.tt_line_cache_hits))
+    assert np.array_equal(out, c1["tables"])
+    if lds:
+        assert st.tt_line_cache_hits > 0.05 * st.nodes
+    else:
+        assert st.tt_line_cache_hits == 0
+
+
 def test_stream_with_net_predictions_while_running(ddx, c1):
     """Seeds predicted one deal at a time on the host while the persistent stream kernel runs
     (the net's copies and kernels must not wait for that kernel): tables exact."""
