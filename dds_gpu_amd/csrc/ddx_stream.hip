@@ -55,8 +55,11 @@ __device__ __forceinline__ long long load_acquire_sys(volatile long long* p) {
 }
 
 // TEAM = 1: one lane per search; 16: a 16-lane team per search. Values the host changes while
-// the kernel runs (produced, eof, stop) are read by one lane and broadcast to its team, so the
-// lanes of a team never disagree.
+// the kernel runs (produced, eof, stop) live in host memory: one lane of the wave reads them,
+// at most every 16 rounds and only while some lane waits for input, and broadcasts them to the
+// wave. A read crosses PCIe, and the whole wave -- its busy lanes too -- waits for it, so
+// per-lane polling every round would slow the searches most when the load is light. The
+// lanes of a team never disagree (they see one broadcast value).
 template <int TEAM>
 __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   using TM = TeamOf<TEAM>;
@@ -70,8 +73,23 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   long long pending = -1;  // job reserved, maybe not yet produced
   u32 published = 0;
   unsigned rounds = 0;
+  long long seen_produced = 0;  // the wave's last view of the host's counters
+  int seen_end = 0;
   for (;;) {
-    if ((++rounds & 1023) == 0 && TM::bcast((int)a.ctl->stop, 0)) L.stage = ST_EXIT;
+    ++rounds;
+    if (((rounds & 1023) == 0 && !seen_end) ||
+        ((rounds & 15) == 0 && __any(L.stage == ST_JOB && (pending < 0 || pending / jpd >= seen_produced)))) {
+      long long v = 0;
+      int e = 0;
+      if (__lane_id() == 0) {  // end flags first: the host sets eof only after its last produced
+        e = (__hip_atomic_load((int*)&a.ctl->eof, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) ? 1 : 0) |
+            (__hip_atomic_load((int*)&a.ctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) ? 2 : 0);
+        v = load_acquire_sys(&a.ctl->produced);
+      }
+      seen_produced = __shfl(v, 0);
+      seen_end = __shfl(e, 0);
+    }
+    if (seen_end & 2) L.stage = ST_EXIT;
     // Cheap stages until the search must enter a node, needs a job, or is done.
     while (L.stage == ST_ROOT || L.stage == ST_NEXT || L.stage == ST_RET) {
       if (L.stage == ST_ROOT) lane_root(L, B);
@@ -94,13 +112,12 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
     }
     if (L.stage == ST_JOB) {
       if (pending < 0) pending = TM::take(&a.counters[0]);
-      const long long p = pending / jpd;
-      const long long produced = TM::bcast(load_acquire_sys(&a.ctl->produced), 0);
-      if (p < produced) {
+      if (pending / jpd < seen_produced) {
+        const long long p = pending / jpd;
         const long long slot = a.work[p % a.work_ring];
         lane_load_job<X>(L, B, slot * jpd + pending % jpd);  // to ROOT; runs next round
         pending = -1;
-      } else if (TM::bcast((int)(a.ctl->eof || a.ctl->stop), 0)) {
+      } else if (seen_end) {
         L.stage = ST_EXIT;
       }
     }
