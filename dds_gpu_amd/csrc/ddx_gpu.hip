@@ -548,7 +548,7 @@ extern "C" int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats) {
   float ms = 0;
   HIPCHECK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
-  unsigned long long c[8];
+  unsigned long long c[16];  // the whole counter block (16 x 8 bytes)
   HIPCHECK(hipMemcpy(c, h->counters, sizeof c, hipMemcpyDeviceToHost));
   if (stats) {
     stats->nodes = c[1];
