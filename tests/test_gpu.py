@@ -194,6 +194,25 @@ def test_stream_with_net_predictions_while_running(ddx, c1):
     assert np.array_equal(np.stack([got[k] for k in range(n)]), c1["tables"][:n])
 
 
+def test_dd_pbn_gpu_mode(c1):
+    """dd_pbn with GPU=1 (one batch on the GPU): leader-side tricks of the C1 endings."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    R = "23456789TJQKA"
+    hands, tables = c1["hands"][:200], c1["tables"][:200]
+    text = "\n".join("N:" + " ".join(".".join("".join(R[r] for r in range(12, -1, -1) if (int(h[p]) >> (16 * s + r)) & 1)
+                                             for s in range(4)) for p in range(4)) for h in hands) + "\n"
+    r = subprocess.run([os.path.join(root, "dds_gpu_amd", "bin", "dd_pbn")], input=text, capture_output=True,
+                       text=True, timeout=200, env=dict(os.environ, GPU="1"))
+    assert r.returncode == 0, r.stderr
+    got = np.array([list(map(int, l.split())) for l in r.stdout.strip().split("\n")]).reshape(-1, 5, 4)
+    t = tables.reshape(-1, 5, 4).astype(int)
+    for leader in range(4):
+        decl = (leader + 3) % 4
+        ns = t[:, :, decl] if decl % 2 == 0 else 13 - t[:, :, decl]
+        assert np.array_equal(got[:, :, leader], ns if leader % 2 == 0 else 8 - ns)
+
+
 def test_bad_configs_fail_cleanly(ddx):
     with pytest.raises(ddx.DdxError):
         ddx.GpuSolver(lanes=100)
