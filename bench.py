@@ -180,8 +180,8 @@ def run_leg(ddx, torch, name, cfg, d_hands, n, a, gold, seeds=None, predict_s=0.
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU: deals shard across GPUs with no exchange (DESIGN.md, "
-              "section 7), but only --gpus 1 is run here", file=sys.stderr)
+        print("bench.py measures one GPU: deals shard across GPUs with no exchange (each GPU "
+              "would take its own range of the batch), but only --gpus 1 is run here", file=sys.stderr)
         sys.exit(2)
     import torch
     from dds_gpu_amd import ddx

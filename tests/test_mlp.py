@@ -107,7 +107,7 @@ def test_gpu_gradients_match_oracle(ddx, batch, masked):
 
 
 @pytest.mark.gpu
-def test_gpu_training_learns_and_predicts(ddx):
+def test_gpu_training_learns_and_predicts(ddx, tmp_path):
     """A few Adam epochs on 1,600 solved deals lower the loss; predictions on held-out deals
     beat the constant predictor and the saved net reloads to the same seeds."""
     g = golden("full_2000.npz")
@@ -123,8 +123,7 @@ def test_gpu_training_learns_and_predicts(ddx):
     const = float((ns == np.bincount(ns.reshape(-1)).argmax()).mean())
     print("held out: exact %.3f, within one %.3f, constant %.3f" % (exact, within1, const))
     assert exact > const
-    path = os.path.join(ROOT, "gpurun_out", "mlp_test.bin")
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    path = str(tmp_path / "mlp_test.bin")
     net.save(path)
     net2 = ddx.TrickNet(seed=9, path=path)
     assert np.array_equal(net2.predict(h[1600:1700]), seeds[:100])
