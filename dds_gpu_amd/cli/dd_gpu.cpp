@@ -13,6 +13,9 @@
 //   MODEL=FILE  a trained trick-count net (dd_train): its predictions seed DDX_SEED_CERT; the
 //               prediction time is counted in the reported time
 //   TEAM=16     sixteen lanes per search (THREADS / 16 searches)
+//   WAVES=3|4   hold the kernel to that many waves per SIMD (split schedule); the tuned
+//               configuration for full deals is THREADS=196608 WAVES=3 with TT_LOG2 11
+//   LDS=3       keep each lane's last 3 TT tag lines in LDS (one lane per search)
 //   WAVE, STACK_KB  accepted for compatibility; the wavefront kernel is the only kernel and
 //               its frames live in HBM, so there is no stack limit to set
 #include <stdio.h>
@@ -72,6 +75,8 @@ int main(int argc, char** argv) {
   ddx_gpu_default_config(&cfg);
   cfg.jobs_per_deal = getenv("PER_LEAD") && atoi(getenv("PER_LEAD")) ? 20 : 5;
   cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
+  cfg.waves_per_simd = getenv("WAVES") ? atoi(getenv("WAVES")) : 0;
+  cfg.lds_lines = getenv("LDS") ? atoi(getenv("LDS")) : 0;
   const long long jobs = n * cfg.jobs_per_deal;
   long long threads = getenv("THREADS") ? atoll(getenv("THREADS")) : 32768;
   if (threads > jobs * cfg.team) threads = jobs * cfg.team;  // spare searches would find no job

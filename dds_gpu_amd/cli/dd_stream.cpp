@@ -15,6 +15,7 @@
 //   RATE         bench: Poisson arrivals at RATE deals/s (default 0 = as fast as slots free)
 //   SEED         bench: deal stream seed (default 2026)
 //   TEAM=16      sixteen lanes per search (THREADS / 16 searches)
+//   WAVES=3      hold the kernel to 3 waves per SIMD
 //   MODEL=FILE   a trained trick-count net: each deal's 20 predictions are computed on the GPU
 //                when it arrives (inside its latency) and seed the search (DDX_SEED_CERT)
 // Bench prints the latency from arrival to result, p50 and p99 in ms, over the deals that
@@ -63,6 +64,7 @@ int main(int argc, char** argv) {
   cfg.tt_log2 = tt_log2;
   cfg.jobs_per_deal = getenv("PER_LEAD") && !atoi(getenv("PER_LEAD")) ? 5 : 20;
   cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
+  cfg.waves_per_simd = getenv("WAVES") ? atoi(getenv("WAVES")) : 0;
   ddx_mlp* net = nullptr;
   if (getenv("MODEL")) {
     uint64_t h0[4];

@@ -61,7 +61,7 @@ __device__ __forceinline__ long long load_acquire_sys(volatile long long* p) {
 // per-lane polling every round would slow the searches most when the load is light. The
 // lanes of a team never disagree (they see one broadcast value).
 template <int TEAM>
-__global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
+__device__ __forceinline__ void stream_body(const StreamArgs& a) {
   using TM = TeamOf<TEAM>;
   using X = typename TM::Ops;
   const long long lane = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / TEAM;
@@ -142,6 +142,16 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   atomicAdd(&a.counters[5], (unsigned long long)L.jobs_done);
 }
 
+// At the compiler's register allocation, or held to 3 waves per SIMD (more searches resident).
+template <int TEAM>
+__global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
+  stream_body<TEAM>(a);
+}
+template <int TEAM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void stream_kernel_w3(StreamArgs a) {
+  stream_body<TEAM>(a);
+}
+
 // ---------------------------------------------------------------------------------------
 struct ddx_stream {
   ddx_gpu_config cfg;
@@ -205,7 +215,7 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   else ddx_gpu_default_config(&cfg);
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 256 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3 ||
-      (cfg.team != 1 && cfg.team != 16)) {
+      (cfg.team != 1 && cfg.team != 16) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3)) {
     ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, 20], "
                   "jobs_per_deal 5 or 20)");
     return DDX_E_ARG;
@@ -268,10 +278,15 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   a.seed_mode = cfg.seed_mode;
   if (!a.hands || !a.seeds || !a.out || !a.work || !a.comp || !a.ctl) return fail("mapped views");
   s->t0 = std::chrono::steady_clock::now();
-  if (cfg.team == 16)
-    hipLaunchKernelGGL(stream_kernel<16>, dim3((unsigned)(cfg.lanes / 256)), dim3(256), 0, s->stream, a);
-  else
-    hipLaunchKernelGGL(stream_kernel<1>, dim3((unsigned)(cfg.lanes / 256)), dim3(256), 0, s->stream, a);
+  const dim3 grid((unsigned)(cfg.lanes / 256));
+  if (cfg.waves_per_simd == 3) {
+    if (cfg.team == 16) hipLaunchKernelGGL(stream_kernel_w3<16>, grid, dim3(256), 0, s->stream, a);
+    else hipLaunchKernelGGL(stream_kernel_w3<1>, grid, dim3(256), 0, s->stream, a);
+  } else if (cfg.team == 16) {
+    hipLaunchKernelGGL(stream_kernel<16>, grid, dim3(256), 0, s->stream, a);
+  } else {
+    hipLaunchKernelGGL(stream_kernel<1>, grid, dim3(256), 0, s->stream, a);
+  }
   if (hipGetLastError() != hipSuccess) {
     ddx_set_error("ddx_stream_open: kernel launch failed");
     stream_free(s);

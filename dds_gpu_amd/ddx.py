@@ -285,10 +285,11 @@ class Stream:
     """The persistent streaming solver (ddx_stream_*): push deals, poll finished tables."""
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=20, seed_mode=SEED_PLAIN,
-                 ring_slots=4096, device=0, team=1):
+                 ring_slots=4096, device=0, team=1, waves_per_simd=0):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2, cfg.team = device, lanes, tt_log2, team
+        cfg.waves_per_simd = waves_per_simd
         cfg.jobs_per_deal, cfg.seed_mode = jobs_per_deal, seed_mode
         self.h = ctypes.c_void_p()
         _check(lib().ddx_stream_open(ctypes.byref(cfg), ring_slots, ctypes.byref(self.h)))

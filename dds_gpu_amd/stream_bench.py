@@ -1,7 +1,7 @@
 """Streaming latency on the local GPU: dd_stream bench at Poisson arrival rates.
 
     python -m dds_gpu_amd.stream_bench --capacity C [--loads 0.5,0.8,1.0,1.2] [--seconds 60]
-        [--lanes 65536] [--team 16] [--tt-log2 12] [--model dds_gpu_amd/models/tricknet.bin]
+        [--lanes 65536] [--team 16] [--tt-log2 12] [--waves 0|3] [--model dds_gpu_amd/models/tricknet.bin]
         [--out FILE.json]
 
 For each load L, dd_stream runs its bench for `seconds` with seeded random full deals arriving
@@ -50,13 +50,14 @@ def main():
     ap.add_argument("--lanes", type=int, default=65536)
     ap.add_argument("--team", type=int, default=16)
     ap.add_argument("--tt-log2", type=int, default=12)
+    ap.add_argument("--waves", type=int, default=0, help="3: kernel held to 3 waves per SIMD")
     ap.add_argument("--model", default=os.path.join(HERE, "models", "tricknet.bin"))
     ap.add_argument("--out", default="")
     a = ap.parse_args()
     results = []
     for load in [float(x) for x in a.loads.split(",")]:
         for seeded in ([False, True] if os.path.exists(a.model) else [False]):
-            env = {"THREADS": a.lanes, "TEAM": a.team, "RATE": load * a.capacity, "PER_LEAD": 1}
+            env = {"THREADS": a.lanes, "TEAM": a.team, "RATE": load * a.capacity, "PER_LEAD": 1, "WAVES": a.waves}
             if seeded:
                 env["MODEL"] = a.model
             print("== load %.2f (%.1f deals/s), %s" % (load, load * a.capacity, "learned" if seeded else "plain"),
@@ -64,7 +65,7 @@ def main():
             r = dict(load=load, rate=round(load * a.capacity, 2), learned=seeded, **run(a.seconds, a.tt_log2, env))
             print(json.dumps(r), flush=True)
             results.append(r)
-    doc = {"capacity": a.capacity, "lanes": a.lanes, "team": a.team, "tt_log2": a.tt_log2,
+    doc = {"capacity": a.capacity, "lanes": a.lanes, "team": a.team, "tt_log2": a.tt_log2, "waves_per_simd": a.waves,
            "seconds": a.seconds, "runs": results}
     if a.out:
         with open(a.out, "w") as f:

@@ -120,13 +120,14 @@ def test_stop_leaves_unsolved_marked(ddx, oracle):
     assert ((part == 255) | (part <= 13)).all()
 
 
-@pytest.mark.parametrize("team,tt_log2", [(1, 10), (16, 12)])
-def test_stream_matches_batch(ddx, c1, team, tt_log2):
+@pytest.mark.parametrize("team,tt_log2,waves", [(1, 10, 0), (16, 12, 0), (16, 12, 3)])
+def test_stream_matches_batch(ddx, c1, team, tt_log2, waves):
     """The persistent streaming kernel: all deals come back, tables exact, ids intact."""
     n = 1500
     hands = c1["hands"][:n]
     got = {}
-    with ddx.Stream(lanes=4096, tt_log2=tt_log2, jobs_per_deal=20, ring_slots=256, team=team) as s:
+    with ddx.Stream(lanes=4096, tt_log2=tt_log2, jobs_per_deal=20, ring_slots=256, team=team,
+                    waves_per_simd=waves) as s:
         i = 0
         t0 = time.time()
         while len(got) < n and time.time() - t0 < 60:
