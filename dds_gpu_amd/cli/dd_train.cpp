@@ -3,7 +3,8 @@
 // DATA.bin: records of 52 bytes, 4 x little-endian uint64 hands (N,E,S,W) + 20 uint8 tricks
 // [strain C,D,H,S,NT][declarer N,E,S,W]. The last HOLDOUT deals (default 2000) are held out; the
 // held-out accuracy of the predicted NS tricks (exact, within one) is printed after each
-// epoch. Defaults: 20 epochs, 512 deals per batch, learning rate 1e-3 (Adam).
+// epoch. Defaults: 20 epochs, 512 deals per batch, learning rate 1e-3 (Adam). INIT=MODEL.bin
+// starts from a saved net instead of a random one (e.g. a second pass at a lower rate).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -43,10 +44,11 @@ int main(int argc, char** argv) {
   printf("%lld deals (%lld train, %lld held out), %d epochs, batch %d deals, lr %g\n", n, ntrain,
          holdout, epochs, batch, lr);
   ddx_mlp* net = nullptr;
-  if (ddx_mlp_create(0, 12345, &net) != DDX_OK) {
+  if (ddx_mlp_create(0, 12345, &net) != DDX_OK || (getenv("INIT") && ddx_mlp_load(net, getenv("INIT")) != DDX_OK)) {
     fprintf(stderr, "%s\n", ddx_last_error());
     return 1;
   }
+  if (getenv("INIT")) printf("starting from %s\n", getenv("INIT"));
   std::vector<uint8_t> seeds(holdout * 20);
   for (int e = 0; e < epochs; e++) {
     float la[2];
