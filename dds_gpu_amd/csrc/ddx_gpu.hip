@@ -23,9 +23,11 @@
 //             (the reference's order, gpu/gpu_main.cu:118-132)
 //   CHAIN     each lane runs its cheap stages (JOB/ROOT/NEXT/RET) back to back until it
 //             reaches ENTER; then the wave enters nodes for all lanes together
-//   SPLIT     as CHAIN, but follow nodes (2nd-4th hand, cheap) are entered in rounds of their
-//             own, so trick-start nodes (TT probe, quick/later tricks) gather and run with
-//             most of the wave active and their HBM probes issued together
+//   SPLIT     as CHAIN, but the chain also enters follow nodes (2nd-4th hand, cheap), so
+//             trick-start nodes (TT probe, quick/later tricks) gather and run with the whole
+//             wave active and their HBM probes issued together
+// CHAIN and SPLIT run a lane's stages in passes of RET, NEXT, JOB, ROOT (and, for SPLIT, a
+// follow node's ENTER), the order in which a lane usually meets them.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -89,8 +91,8 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   unsigned long long* next = a.counters;
   auto take = [&]() -> long long { return TM::take(next); };
   // Profile (PROF only; wave-uniform counters kept by every lane, lane 0 reports):
-  // kinds 0..5 = stage rounds of the PRIORITY schedule, 6 = cheap chains, 7 = follow ENTER,
-  // 2 = (trick-start) ENTER in the other schedules.
+  // kinds 0..5 = stage rounds of the PRIORITY schedule, 6 = cheap chains (with the follow
+  // nodes' ENTER under SPLIT), 2 = (trick-start) ENTER in the other schedules.
   unsigned long long p_rounds[8], p_lanes[8], p_cycles[8];
   unsigned p_exit[EX_COUNT];
   if (PROF) {
@@ -109,22 +111,24 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   u32 reported = 0;  // finished jobs already added to the progress counter
   u64 reported_nodes = 0;
 
-  // Cheap stages of one lane, back to back, until it must enter a node or has no work.
+  // Cheap stages of one lane, back to back, until it must enter a node or has no work. Each
+  // pass runs the stages in the order a lane usually meets them -- RET, NEXT, JOB, ROOT -- so
+  // that a child's return and the next move, or a search's end and the next one's start, take
+  // one pass rather than one per stage. Under SPLIT a pass also enters follow nodes (2nd-4th
+  // hand), so the chain runs until every lane waits at a trick start (or has no work).
   auto cheap_chain = [&]() {
-    while (L.stage != ST_ENTER && L.stage != ST_EXIT) {
+    while (L.stage != ST_EXIT && (L.stage != ST_ENTER || (SCHED == DDX_SCHED_SPLIT && L.p.nplayed != 0))) {
+      if (L.stage == ST_RET) lane_ret<X>(L, B, st);
+      if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
       if (L.stage == ST_JOB) {
         const long long job = take();
-        if (job >= a.jobs) {
-          L.stage = ST_EXIT;
-          break;
-        }
-        lane_load_job<X>(L, B, job);
-      } else if (L.stage == ST_ROOT) {
-        lane_root(L, B);
-      } else if (L.stage == ST_NEXT) {
-        lane_next<X>(L, B, st);
-      } else {
-        lane_ret<X>(L, B, st);
+        if (job >= a.jobs) L.stage = ST_EXIT;
+        else lane_load_job<X>(L, B, job);
+      }
+      if (L.stage == ST_ROOT) lane_root(L, B);
+      if (SCHED == DDX_SCHED_SPLIT && L.stage == ST_ENTER && L.p.nplayed != 0) {
+        lane_enter<X>(L, B, st);
+        if (PROF) p_exit[L.exit_at]++;
       }
     }
   };
@@ -162,17 +166,7 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
       cheap_chain();
       prof_round(6, cheap, t0);
       if (__all(L.stage == ST_EXIT)) break;
-      if (SCHED == DDX_SCHED_SPLIT && __any(L.stage == ST_ENTER && L.p.nplayed != 0)) {
-        // Follow nodes first; trick-start lanes wait and gather.
-        const bool run = L.stage == ST_ENTER && L.p.nplayed != 0;
-        t0 = PROF ? clock64() : 0;
-        if (run) {
-          lane_enter<X>(L, B, st);
-          if (PROF) p_exit[L.exit_at]++;
-        }
-        prof_round(7, run, t0);
-        continue;
-      }
+      // The trick-start nodes (SPLIT), or every node (CHAIN), entered with the wave gathered.
       const bool run = L.stage == ST_ENTER;
       t0 = PROF ? clock64() : 0;
       if (run) {

@@ -21,8 +21,8 @@ import numpy as np
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 from dds_gpu_amd import ddx  # noqa: E402
 
-KINDS = {0: "JOB", 1: "ROOT", 2: "ENTER (trick start)", 3: "NEXT", 4: "RET", 6: "cheap-stage chain",
-         7: "ENTER (follow)"}
+KINDS = {0: "JOB", 1: "ROOT", 2: "ENTER (trick start)", 3: "NEXT", 4: "RET",
+         6: "cheap-stage chain (with follow-node ENTER under SPLIT)"}
 EXITS = ["trivial bound", "last trick", "TT", "quick tricks", "later tricks", "2nd-hand quick tricks",
          "moves generated"]
 

@@ -132,7 +132,8 @@ typedef struct ddx_gpu_config {
 enum {
   DDX_SCHED_PRIORITY = 0, /* one stage per round: RET > NEXT > ROOT > JOB > ENTER (reference) */
   DDX_SCHED_CHAIN = 1,    /* cheap stages run back to back per lane, then ENTER for all */
-  DDX_SCHED_SPLIT = 2     /* as CHAIN, and follow nodes enter before trick-start nodes */
+  DDX_SCHED_SPLIT = 2     /* as CHAIN, with follow nodes entered inside the chain, so that
+                             trick-start nodes are entered with the whole wave gathered */
 };
 
 typedef struct ddx_gpu ddx_gpu;
@@ -169,8 +170,9 @@ int ddx_gpu_progress_nodes(ddx_gpu* h, uint64_t* jobs_done, uint64_t* nodes);
 /* Duration in ms of the last solve kernel, from HIP events on its stream. */
 double ddx_gpu_last_kernel_ms(const ddx_gpu* h);
 /* Per-stage counters of the last solves (config.profile = 1), per wave64: out32[0..7] rounds by
- * kind (0 JOB, 1 ROOT, 2 ENTER -- trick-start ENTER under CHAIN/SPLIT --, 3 NEXT, 4 RET,
- * 6 cheap-stage chains, 7 follow ENTER), out32[8..15] active lanes summed over those rounds,
+ * kind (0 JOB, 1 ROOT, 2 ENTER -- trick-start ENTER under SPLIT, every ENTER under CHAIN --,
+ * 3 NEXT, 4 RET, 6 cheap-stage chains, with the follow nodes' ENTER under SPLIT; 7 unused),
+ * out32[8..15] active lanes summed over those rounds,
  * out32[16..22] where ENTER decided nodes (trivial, last trick, TT, quick tricks, later tricks,
  * 2nd-hand quick tricks, moves), out32[24..31] shader-clock cycles by kind. reset != 0 clears. */
 int ddx_gpu_profile(ddx_gpu* h, uint64_t* out32, int reset);
