@@ -67,8 +67,8 @@ def candidates(quick):
         for tt, sched, jpd in itertools.product(tts, (0, 1, 2), (5, 20)):
             if (quick or w) and sched != 2:
                 continue
-            if quick and (tpc in (256, 1024) or (team == 16 and jpd == 20)):
-                continue  # measured far behind: one wave per SIMD; spills at four; team per lead
+            if quick and (tpc == 256 or team == 16 or jpd == 20):
+                continue  # measured far behind on full deals: one wave per SIMD; teams; jobs per lead
             if team == 1 and tpc * CUS * 320 * (1 << tt) > 180e9:
                 continue  # keep the TTs within ~180 GB of the 288 GB
             out.append({"lanes": CUS * tpc, "team": team, "tt_log2": tt, "schedule": sched,
