@@ -142,13 +142,6 @@ struct GlobalLines {
   DDX_M void store(const TTView& tt, u32 b, const TagLine& L) const { store_line(tt, b, L); }
 };
 
-// Mask of tag positions whose tag equals k.tag apart from the way bits.
-DDX_FN u32 match_mask(const TagLine& L, u32 tag) {
-  u32 m = 0;
-#pragma unroll
-  for (int i = 0; i < kWays; i++) m |= (u32)((L.t[i] & ~0xF00u) == tag) << i;
-  return m;
-}
 DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, by masks (see seat_hand)
   u32 v = 0;
 #pragma unroll
@@ -156,32 +149,79 @@ DDX_FN u32 tag_at(const TagLine& L, int pos) {  // run-time position, by masks (
   return v;
 }
 
+// Ways of the tags that equal `tag` apart from their way bits, newest first, 4 bits each
+// (the first in bits 0..3); returns how many there are.
+DDX_FN int match_ways(const TagLine& L, u32 tag, u64& ways) {
+  u64 w = 0;
+  int n = 0;
+#pragma unroll
+  for (int i = 0; i < kWays; i++) {
+    const bool hit = (L.t[i] & ~0xF00u) == tag;
+    w |= hit ? (u64)((L.t[i] >> 8) & 15) << (4 * n) : 0;
+    n += hit ? 1 : 0;
+  }
+  ways = w;
+  return n;
+}
+
+// The entries of the next N ways of `ways`, read with their loads issued together (one
+// memory round trip for N dependent ones); ways past the matching ones read way 0, which is
+// never examined. pop() returns the front entry and moves the others up one place (no
+// run-time array index, so the N stay in registers).
+template <int N>
+struct EntryQueue {
+  Entry e[N];
+  DDX_M void load(const Entry* base, u64 ways) {
+#pragma unroll
+    for (int j = 0; j < N; j++) e[j] = base[(ways >> (4 * j)) & 15];
+  }
+  DDX_M Entry pop() {
+    const Entry f = e[0];
+#pragma unroll
+    for (int j = 0; j + 1 < N; j++) e[j] = e[j + 1];
+    return f;
+  }
+};
+// Entries read together by a probe and by a store. Measured on gfx950 (one lane per search,
+// full deals): 4 and 2 give 14% more nodes/s than one at a time; wider batches in the store
+// cost registers (the store also keeps the tag line live) for no gain.
+constexpr int kProbeBatch = 4;
+constexpr int kStoreBatch = 2;
+
 // Bounds lookup: 1 = NS reach `need` more tricks, 0 = they do not (rel = the relevant top
 // cards of the entry), -1 = unknown; then `hint` may get a stored best card. Matching tags
 // are visited newest first; a later matching entry's best card replaces an earlier one
-// (reference dd_tt_probe, src/dd.h:458-493).
+// (reference dd_tt_probe, src/dd.h:458-493). The entries are read kProbeBatch at a time and still
+// examined one by one in that order, so the result and `ent_reads` (entries examined) are
+// those of a one-by-one read.
 template <class Lines = GlobalLines>
 DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hint, u64& ent_reads,
                     const Lines& lines = Lines()) {
   TagLine L;
   lines.load(tt, k.bucket, L);
-  u32 m = match_mask(L, k.tag);
+  u64 ways;
+  int n = match_ways(L, k.tag, ways);
   const Entry* base = tt.ents + (size_t)k.bucket * kWays;
-  while (m) {
-    int pos = __builtin_ctz(m);
-    m &= m - 1;
-    Entry e = base[(tag_at(L, pos) >> 8) & 15];
-    ent_reads++;
-    if ((e.w0 & kQMask) != k.q) continue;
-    u64 r;
-    if (!entry_matches(k, e.w0, e.w1, r)) continue;
-    int lb = (int)(e.w0 >> kQBits) & 15, ub = (int)(e.w0 >> (kQBits + 4)) & 15;
-    if (lb >= need || ub < need) {
-      rel = r;
-      return lb >= need ? 1 : 0;
+  while (n > 0) {
+    EntryQueue<kProbeBatch> q;
+    q.load(base, ways);
+    const int cnt = n < kProbeBatch ? n : kProbeBatch;
+    for (int j = 0; j < cnt; j++) {
+      const Entry e = q.pop();
+      ent_reads++;
+      if ((e.w0 & kQMask) != k.q) continue;
+      u64 r;
+      if (!entry_matches(k, e.w0, e.w1, r)) continue;
+      int lb = (int)(e.w0 >> kQBits) & 15, ub = (int)(e.w0 >> (kQBits + 4)) & 15;
+      if (lb >= need || ub < need) {
+        rel = r;
+        return lb >= need ? 1 : 0;
+      }
+      int b = (int)(e.w0 >> (kQBits + 8)) & 63;
+      if (b != 63) hint = b;
     }
-    int b = (int)(e.w0 >> (kQBits + 8)) & 63;
-    if (b != 63) hint = b;
+    ways >>= 4 * kProbeBatch;
+    n -= kProbeBatch;
   }
   return -1;
 }
@@ -230,21 +270,29 @@ DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, in
   TagLine L;
   lines.load(tt, k.bucket, L);
   Entry* base = tt.ents + (size_t)k.bucket * kWays;
-  u32 m = match_mask(L, k.tag);
-  while (m) {
-    int pos = __builtin_ctz(m);
-    m &= m - 1;
-    Entry* e = base + ((tag_at(L, pos) >> 8) & 15);
-    u64 w0 = e->w0;
-    if ((w0 & kQMask) == k.q && e->w1 == w1 && (w0 >> 53 << 53) == hi) {
-      int olb = (int)(w0 >> kQBits) & 15, oub = (int)(w0 >> (kQBits + 4)) & 15;
-      int obest = (int)(w0 >> (kQBits + 8)) & 63;
-      if (olb > lb) lb = olb;
-      if (oub < ub) ub = oub;
-      if (best < 0 && obest != 63) best = obest;
-      e->w0 = pack_w0(k.q, lb, ub, best, hi);
-      return;
+  u64 ways;
+  int n = match_ways(L, k.tag, ways);
+  while (n > 0) {  // matching entries kStoreBatch at a time, examined newest first
+    EntryQueue<kStoreBatch> q;
+    q.load(base, ways);
+    const int cnt = n < kStoreBatch ? n : kStoreBatch;
+    for (int j = 0; j < cnt; j++) {
+      const Entry e = q.pop();
+      const int w = (int)(ways & 15);
+      ways >>= 4;
+      const u64 w0 = e.w0;
+      if ((w0 & kQMask) == k.q && e.w1 == w1 && (w0 >> 53 << 53) == hi) {
+        int olb = (int)(w0 >> kQBits) & 15, oub = (int)(w0 >> (kQBits + 4)) & 15;
+        int obest = (int)(w0 >> (kQBits + 8)) & 63;
+        if (olb > lb) lb = olb;
+        if (oub < ub) ub = oub;
+        if (best < 0 && obest != 63) best = obest;
+        base[w].w0 = pack_w0(k.q, lb, ub, best, hi);
+        return;
+      }
     }
+    ways >>= 4 * (kStoreBatch - cnt);
+    n -= kStoreBatch;
   }
   int way = take_way(L, k.tag, gen);
   lines.store(tt, k.bucket, L);
