@@ -50,7 +50,8 @@ sys.path.insert(0, ROOT)
 METRIC = "full-deal DD tables/sec on 1 MI355X, learned-seeded vs plain, bit-exact; p99 ms"
 STREAM_SEED = 2026
 HBM_PEAK_GBS = 8000.0
-PLAIN = {"lanes": 65536, "team": 1, "tt_log2": 10, "jobs_per_deal": 5, "schedule": 2, "waves_per_simd": 0}
+PLAIN = {"lanes": 65536, "team": 1, "tt_log2": 10, "jobs_per_deal": 5, "schedule": 2, "waves_per_simd": 0,
+         "block": 256}
 DEFAULT_MODEL = os.path.join(ROOT, "dds_gpu_amd", "models", "tricknet.bin")
 DEFAULT_TUNED = os.path.join(ROOT, "dds_gpu_amd", "tuned.json")
 
@@ -132,6 +133,7 @@ def run_leg(ddx, torch, name, cfg, d_hands, n, a, gold, seeds=None, predict_s=0.
     stream = torch.cuda.current_stream().cuda_stream
     g = ddx.GpuSolver(lanes=cfg["lanes"], tt_log2=cfg["tt_log2"], jobs_per_deal=cfg["jobs_per_deal"],
                       schedule=cfg["schedule"], team=cfg["team"], waves_per_simd=cfg["waves_per_simd"],
+                      block=cfg.get("block", 256),
                       seed_mode=ddx.SEED_CERT if seeds is not None else ddx.SEED_PLAIN)
     jpd = cfg["jobs_per_deal"]
     log("[%s] %s, %.1f GB of lane state" % (name, json.dumps(cfg), g.device_bytes / 1e9))

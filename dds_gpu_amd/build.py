@@ -4,7 +4,7 @@
 
 Outputs (git-ignored, shipped with the tree):
     dds_gpu_amd/lib/libddx.so            the C ABI of include/ddx.h
-    dds_gpu_amd/bin/dd_cpu, dd_gpu, dd_stream, dd_train, dd_pbn, pbn   command lines
+    dds_gpu_amd/bin/dd_cpu, dd_gpu, dd_stream, dd_train, dd_pbn, pbn, dd_verify   command lines
 """
 import os
 import shutil
@@ -31,7 +31,7 @@ HOST_ARCH = ["-march=x86-64-v3"]
 
 HIP_SOURCES = ["ddx_gpu.hip", "ddx_mlp.hip", "ddx_stream.hip"]
 HOST_SOURCES = ["ddx_host.cpp"]
-TOOLS = ["dd_cpu", "dd_gpu", "dd_stream", "dd_train", "dd_pbn", "pbn"]
+TOOLS = ["dd_cpu", "dd_gpu", "dd_stream", "dd_train", "dd_pbn", "pbn", "dd_verify"]
 
 
 def _run(cmd, verbose):

@@ -23,6 +23,10 @@
 #pragma once
 #include "search.hpp"
 
+#if DDX_VERIFY
+#include <stdio.h>
+#endif
+
 // DDX_HELP = 1 compiles the hooks for tail helpers (dds_gpu_amd/csrc/ddx_help.hpp, which sets
 // it); without it they fold away and the machine is exactly the plain one.
 #ifndef DDX_HELP
@@ -99,6 +103,10 @@ DDX_FN void lane_init(Lane& L, TTView tt) {
   L.c.tt = tt;
   L.c.gen = 0;
   L.c.nodes = L.c.probes = L.c.hits = L.c.ent_reads = L.c.stores = 0;
+#if DDX_VERIFY
+  L.c.no_tt = 0;
+  L.c.verified = L.c.bad = 0;
+#endif
   L.depth = 0;
   L.stage = ST_JOB;
   L.v = 0;
@@ -329,5 +337,40 @@ struct ArrayStack {
   DDX_M void store(int i, const Frame& x) { f[i] = pack_frame(x); }
   DDX_M Frame load(int i) const { return unpack_frame(f[i]); }
 };
+
+
+#if DDX_VERIFY
+// Re-checks a TT hit: the same position and target searched by the same machine with no TT
+// (neither probes nor stores; quick tricks, later tricks and the move order as usual). One
+// search decides it: [lo, hi] = [target - 1, target]. A contradiction is counted and printed
+// in the reference's form (src/dd.h:651-663).
+inline void verify_hit(Ctx& c, const Pos& p, int target, int hit, u64 rel) {
+  Lane V;
+  lane_init(V, c.tt);
+  V.c.no_tt = 1;
+  V.c.verified = V.c.bad = 0;
+  V.c.gen = c.gen;
+  V.p = p;
+  V.target = target;
+  V.lo = target - 1;
+  V.hi = target;
+  V.deal = 0;
+  V.strain = V.decl = V.last_decl = 0;
+  V.stage = ST_ENTER;
+  uint8_t out[20];
+  const Batch B{p.hand, out, nullptr, 1, 5, SEED_PLAIN};
+  ArrayStack st;
+  auto none = [&]() -> long long { return B.jobs; };
+  while (V.stage != ST_JOB && V.stage != ST_EXIT) lane_step(V, B, st, none);
+  const bool truth = V.lo >= target;
+  c.verified++;
+  if (truth != (hit != 0)) {
+    c.bad++;
+    printf("BAD TT HIT: leader %d need %d left %d ns_won %d hit %d truth %d rel %llx\n", p.leader,
+           target - p.ns_won, p.left, p.ns_won, hit, truth ? 1 : 0, (unsigned long long)rel);
+    for (int h = 0; h < 4; h++) printf("  hand %d %llx\n", h, (unsigned long long)p.hand[h]);
+  }
+}
+#endif
 
 }  // namespace ddx

@@ -103,6 +103,12 @@ DDX_FN int move_at(const Frame& f, int i) {
   return i < 10 ? (int)((f.mv >> (6 * i)) & 63) : (int)((f.mv2 >> (6 * (i - 10))) & 63);
 }
 
+// DDX_VERIFY = 1 (host builds only, the dd_verify tool) re-checks every TT hit with a search of
+// the same position and target that uses no TT (reference -DDD_VERIFY, src/dd.h:651-663).
+#ifndef DDX_VERIFY
+#define DDX_VERIFY 0
+#endif
+
 // Per-lane solver context: its TT, the generation of the current solve, counters.
 struct Ctx {
   TTView tt;
@@ -110,7 +116,20 @@ struct Ctx {
   u64 nodes;
   u64 probes, hits;
   u64 ent_reads, stores;  // TT entries read by probes; TT stores (algorithmic HBM bytes)
+#if DDX_VERIFY
+  int no_tt;              // this context's searches neither probe nor store
+  u64 verified, bad;      // TT hits re-checked; hits the TT-free search contradicted
+#endif
 };
+
+// Whether the context leaves the TT alone (only a verifying search does).
+#if DDX_VERIFY
+DDX_FN bool tt_off(const Ctx& c) { return c.no_tt != 0; }
+struct Pos;
+inline void verify_hit(Ctx& c, const Pos& p, int target, int hit, u64 rel);
+#else
+DDX_FN bool tt_off(const Ctx&) { return false; }
+#endif
 
 // ---------------------------------------------------------------------------------------
 // Move generation: one candidate per run of equivalent cards, best first by the DDS
@@ -250,11 +269,14 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   Key key;
   make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
   int hint = -1;
-  if (!DDX_NO_TT) {
+  if (!DDX_NO_TT && !tt_off(c)) {
     int hit = X::probe(c.tt, key, need, rel, hint, c.ent_reads);
     c.probes++;
     c.hits += hit >= 0;
     exit_at = EX_TT;
+#if DDX_VERIFY
+    if (hit >= 0) verify_hit(c, p, target, hit, rel);
+#endif
     if (hit >= 0) return hit;
   }
   Top t;
@@ -268,8 +290,10 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (DDX_NO_QT) decided = false;
   if (decided) {
     const bool val = ns_lead ? q != 0 : q == 0;
-    X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
-    c.stores++;
+    if (!tt_off(c)) {
+      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+      c.stores++;
+    }
     rel = wr;
     exit_at = EX_QT;
     return val ? 1 : 0;
@@ -279,8 +303,10 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
                           : later_max(t, p.trump, p.leader, p.ns_won, p.left, target, wr);
   if (!DDX_NO_LT && lt) {
     const bool val = !ns_lead;
-    X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
-    c.stores++;
+    if (!tt_off(c)) {
+      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+      c.stores++;
+    }
     rel = wr;
     exit_at = EX_LT;
     return val ? 1 : 0;
@@ -397,7 +423,7 @@ DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
     const u64 other = below & ~own;
     acc |= other ? below & ~((2ull << msb(other)) - 1) : below;
   }
-  if (p.nplayed == 0 && !DDX_NO_TT) {
+  if (p.nplayed == 0 && !DDX_NO_TT && !tt_off(c)) {
     Key key;
     make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
     int lb = f.lb, ub = f.ub;
