@@ -116,8 +116,9 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   // that a child's return and the next move, or a search's end and the next one's start, take
   // one pass rather than one per stage. Under SPLIT a pass also enters follow nodes (2nd-4th
   // hand), so the chain runs until every lane waits at a trick start (or has no work).
-  auto cheap_chain = [&]() {
+  auto cheap_chain = [&](unsigned& passes) {
     while (L.stage != ST_EXIT && (L.stage != ST_ENTER || (SCHED == DDX_SCHED_SPLIT && L.p.nplayed != 0))) {
+      if (PROF) passes++;
       if (L.stage == ST_RET) lane_ret<X>(L, B, st);
       if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
       if (L.stage == ST_JOB) {
@@ -162,9 +163,22 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
       prof_round(pick, run, t0);
     } else {
       long long t0 = PROF ? clock64() : 0;
-      const bool cheap = L.stage != ST_ENTER && L.stage != ST_EXIT;
-      cheap_chain();
-      prof_round(6, cheap, t0);
+      unsigned passes = 0;
+      cheap_chain(passes);
+      if (PROF) {
+        // Each pass runs for the lanes still in the chain: the wave's rounds are the most passes
+        // any lane made, its active lanes the passes of all lanes.
+        unsigned most = passes, all = passes;
+#pragma unroll
+        for (int m = 32; m; m >>= 1) {
+          const unsigned o = __shfl_xor(most, m);
+          most = o > most ? o : most;
+          all += __shfl_xor(all, m);
+        }
+        p_rounds[6] += most;
+        p_lanes[6] += all;
+        p_cycles[6] += (unsigned long long)(clock64() - t0);
+      }
       if (__all(L.stage == ST_EXIT)) break;
       // The trick-start nodes (SPLIT), or every node (CHAIN), entered with the wave gathered.
       const bool run = L.stage == ST_ENTER;
