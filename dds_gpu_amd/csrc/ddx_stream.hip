@@ -90,14 +90,13 @@ __device__ __forceinline__ void stream_body(const StreamArgs& a) {
       seen_end = __shfl(e, 0);
     }
     if (seen_end & 2) L.stage = ST_EXIT;
-    // Cheap stages, in passes of RET, NEXT, ROOT and a follow node's ENTER (the order a lane
-    // usually meets them), until the search waits at a trick start, needs a job or is done.
-    while (L.stage == ST_ROOT || L.stage == ST_NEXT || L.stage == ST_RET ||
-           (L.stage == ST_ENTER && L.p.nplayed != 0)) {
-      if (L.stage == ST_RET) lane_ret<X>(L, B, st);
-      if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
+    // Cheap stages until the search must enter a node, needs a job, or is done. (The batch
+    // kernel's fused passes, follow-node ENTER inside the chain, measured 14% lower capacity
+    // here with 16-lane teams: 77.6 against 90.1 deals/s.)
+    while (L.stage == ST_ROOT || L.stage == ST_NEXT || L.stage == ST_RET) {
       if (L.stage == ST_ROOT) lane_root(L, B);
-      if (L.stage == ST_ENTER && L.p.nplayed != 0) lane_enter<X>(L, B, st);
+      else if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
+      else lane_ret<X>(L, B, st);
     }
     if (L.jobs_done != published) {  // a job just finished: count it, maybe publish the deal
       published = L.jobs_done;
@@ -127,7 +126,11 @@ __device__ __forceinline__ void stream_body(const StreamArgs& a) {
     // Every ballot below runs with the whole wave (no divergent continue above), so __all
     // and __any see all 64 lanes.
     if (__all(L.stage == ST_EXIT)) break;
-    if (__any(L.stage == ST_ENTER)) {  // trick starts, entered together
+    if (__any(L.stage == ST_ENTER && L.p.nplayed != 0)) {
+      if (L.stage == ST_ENTER && L.p.nplayed != 0) lane_enter<X>(L, B, st);
+      continue;
+    }
+    if (__any(L.stage == ST_ENTER)) {
       if (L.stage == ST_ENTER) lane_enter<X>(L, B, st);
       continue;
     }

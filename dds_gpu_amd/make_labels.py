@@ -2,6 +2,7 @@
 
     python -m dds_gpu_amd.make_labels OUT.bin --seed 7001 --first 0 --deals 20000 [--gpu]
                                       [--time-limit S] [--partial] [--from-npz DIR]
+                                      [--lanes L] [--waves 0|3] [--tt-log2 T] [--jobs-per-deal 5|20]
 
 Deals come from the seeded generator (stream --seed, indices [first, first + deals)); tables
 from the exact solver (the GPU, or the CPU). With --time-limit the solve stops after S seconds:
@@ -44,6 +45,7 @@ def main():
     ap.add_argument("--team", type=int, default=1)
     ap.add_argument("--tt-log2", type=int, default=12)
     ap.add_argument("--jobs-per-deal", type=int, default=20)
+    ap.add_argument("--waves", type=int, default=0, help="3: the kernel held to 3 waves per SIMD")
     a = ap.parse_args()
     if a.from_npz:
         files = sorted(glob.glob(os.path.join(a.from_npz, "*.npz")))
@@ -55,7 +57,7 @@ def main():
     hands = ddx.random_deals(a.seed, a.deals, 13, a.first)
     if a.gpu:
         with ddx.GpuSolver(lanes=a.lanes, tt_log2=a.tt_log2, team=a.team, jobs_per_deal=a.jobs_per_deal,
-                           schedule=2) as g:
+                           schedule=2, waves_per_simd=a.waves) as g:
             res = {}
             worker = threading.Thread(target=lambda: res.update(r=g.solve(hands, allow_stop=True)))
             worker.start()
