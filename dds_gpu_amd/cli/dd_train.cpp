@@ -1,7 +1,8 @@
 // dd_train: trains the trick-count net on solved deals, on the GPU.
 //   dd_train DATA.bin MODEL.bin [EPOCHS] [BATCH_DEALS] [LR] [HOLDOUT]
 // DATA.bin: records of 52 bytes, 4 x little-endian uint64 hands (N,E,S,W) + 20 uint8 tricks
-// [strain C,D,H,S,NT][declarer N,E,S,W]. The last HOLDOUT deals (default 2000) are held out; the
+// [strain C,D,H,S,NT][declarer N,E,S,W], 255 = unsolved (no label: skipped by the loss and the
+// held-out score). The last HOLDOUT deals (default 2000) are held out; the
 // held-out accuracy of the predicted NS tricks (exact, within one) is printed after each
 // epoch. Defaults: 20 epochs, 512 deals per batch, learning rate 1e-3 (Adam). INIT=MODEL.bin
 // starts from a saved net instead of a random one (e.g. a second pass at a lower rate).
@@ -57,17 +58,19 @@ int main(int argc, char** argv) {
       fprintf(stderr, "%s\n", ddx_last_error());
       return 1;
     }
-    long long exact = 0, near = 0;
+    long long exact = 0, near = 0, known = 0;
     for (long long i = 0; i < holdout; i++)
       for (int j = 0; j < 20; j++) {
         const int t = tables[(ntrain + i) * 20 + j];
+        if (t == 255) continue;  // unsolved: no label
         const int ns = (j & 1) == 0 ? t : 13 - t;
         const int d = (int)seeds[i * 20 + j] - ns;
         exact += d == 0;
         near += d >= -1 && d <= 1;
+        known++;
       }
-    printf("epoch %2d: train loss %.4f acc %.3f | held out exact %.3f within one %.3f\n", e + 1, la[0],
-           la[1], exact / (20.0 * holdout), near / (20.0 * holdout));
+    printf("epoch %2d: train loss %.4f acc %.3f | held out exact %.3f within one %.3f (%lld answers)\n", e + 1,
+           la[0], la[1], exact / (double)(known ? known : 1), near / (double)(known ? known : 1), known);
     fflush(stdout);
   }
   if (ddx_mlp_save(net, argv[2]) != DDX_OK) {
