@@ -8,6 +8,7 @@
 // of a uint64_t hand mask; suit 0..3 = S,H,D,C; rank 0..12 = 2..A. Seats 0..3 = N,E,S,W.
 // Strains 0..4 = C,D,H,S,NT.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 // DDX_FN: free functions; DDX_M: member functions.

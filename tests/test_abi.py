@@ -43,3 +43,20 @@ def test_no_gpu_calls_without_a_device(ddx):
     h = ctypes.c_void_p()
     rc = ddx.lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(h))
     assert rc != 0 and ddx.lib().ddx_last_error()
+
+
+def test_headers_compile_on_their_own(tmp_path):
+    """Each header of the header-only core, and the C ABI header (as C99 and as C++), compiles
+    with nothing included before it."""
+    inc = os.path.join(ROOT, "include")
+    for h in sorted(os.listdir(os.path.join(inc, "ddx"))):
+        src = tmp_path / ("use_%s.cpp" % h.replace(".", "_"))
+        src.write_text('#include "ddx/%s"\n' % h)
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wno-unknown-pragmas", "-I" + inc, str(src)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, h + "\n" + r.stderr
+    for lang, cc, ext in [("c99", "gcc", "c"), ("c++17", "g++", "cpp")]:
+        src = tmp_path / ("use_ddx_h." + ext)
+        src.write_text('#include "ddx.h"\n')
+        r = subprocess.run([cc, "-std=" + lang, "-fsyntax-only", "-I" + inc, str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, lang + "\n" + r.stderr
