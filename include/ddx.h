@@ -113,7 +113,8 @@ typedef struct ddx_gpu_config {
   int64_t lanes;        /* concurrent solver lanes, each with its own TT (default 65536) */
   int tt_log2;          /* buckets per lane TT, >= 9 (default 10) */
   int jobs_per_deal;    /* 5 or 20 */
-  int schedule;         /* stage schedule of the wavefront loop, see DDX_SCHED_* */
+  int schedule;         /* stage schedule of the batch kernel's wavefront loop, see DDX_SCHED_*
+                           (the stream kernel has a loop of its own) */
   int seed_mode;        /* DDX_SEED_* */
   int block;            /* threads per block (multiple of 64; default 256) */
   int profile;          /* 1: collect per-stage counters (slower) */
