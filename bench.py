@@ -255,7 +255,8 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        sample = a.cpu_sample if a.cpu_sample > 0 else min(m, 8 * cores)
+        # ~24 full deals per host thread: 10-15 s of the reference solver's work on its cores
+        sample = a.cpu_sample if a.cpu_sample > 0 else min(m, 24 * cores)
         cpu = cpu_baseline(ddx, hands, gold["tables"], sample)
     wrong = sum(r["bit_exact"]["wrong"] for r in results.values())
     checked = sum(r["bit_exact"]["checked_answers"] for r in results.values())
