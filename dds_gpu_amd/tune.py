@@ -21,7 +21,7 @@ is a bug and aborts the sweep).
 
 Swept: lanes (threads per CU x 256 CUs, with the kernel held to 3 or 4 waves per SIMD where
 the threads need it), lanes per search (one lane, or a 16-lane team), TT_LOG2
-per search (one lane: 10..12, i.e. 0.3-1.3 MB of TT per search and up to 86 GB in all; a team:
+per search (one lane: 9..12, i.e. 0.16-1.3 MB of TT per search and up to 172 GB in all; a team:
 13..14), jobs per deal (5 = one job per strain, 20 = one per lead) and the stage schedule
 (priority / chain / split). The best is written to `out` under the workload's name; bench.py
 and the tools read the "full" entry.
@@ -63,7 +63,7 @@ def candidates(quick):
     1024 need the kernels held to 3 or 4 waves per SIMD."""
     out = []
     for team, tpc, w in [(1, 256, 0), (1, 512, 0), (1, 768, 3), (1, 1024, 4), (16, 512, 0), (16, 768, 3)]:
-        tts = (10, 11, 12) if team == 1 else (13, 14)
+        tts = (9, 10, 11, 12) if team == 1 else (13, 14)
         for tt, sched, jpd in itertools.product(tts, (0, 1, 2), (5, 20)):
             if (quick or w) and sched != 2:
                 continue
