@@ -9,8 +9,8 @@
 // Same argv, records, stderr lines and exit codes as the reference's gpu/stream_main.cu
 // (0 = no mismatch, 1 = mismatch or GPU error, 2 = usage / TT_LOG2 < 9). Environment:
 //   THREADS      solver lanes (default 32768, rounded up to 256)
-//   PER_LEAD     1 (default here): one job per lead; 0: one per strain. With the 4,096-slot
-//                ring, per-strain jobs would leave most lanes without work
+//   PER_LEAD     0 (default, as the reference): one job per strain; 1: one job per lead,
+//                which keeps four times as many lanes busy when the ring holds few deals
 //   RING         ring slots (default 4096)
 //   RATE         bench: Poisson arrivals at RATE deals/s (default 0 = as fast as slots free)
 //   SEED         bench: deal stream seed (default 2026)
@@ -62,7 +62,7 @@ int main(int argc, char** argv) {
   long long threads = getenv("THREADS") ? atoll(getenv("THREADS")) : 32768;
   cfg.lanes = (threads + 255) / 256 * 256;
   cfg.tt_log2 = tt_log2;
-  cfg.jobs_per_deal = getenv("PER_LEAD") && !atoi(getenv("PER_LEAD")) ? 5 : 20;
+  cfg.jobs_per_deal = getenv("PER_LEAD") && atoi(getenv("PER_LEAD")) ? 20 : 5;
   cfg.team = getenv("TEAM") ? atoi(getenv("TEAM")) : 1;
   cfg.waves_per_simd = getenv("WAVES") ? atoi(getenv("WAVES")) : 0;
   ddx_mlp* net = nullptr;

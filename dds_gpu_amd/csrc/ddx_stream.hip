@@ -77,7 +77,7 @@ __device__ __forceinline__ void stream_body(const StreamArgs& a) {
   int seen_end = 0;
   for (;;) {
     ++rounds;
-    if (((rounds & 1023) == 0 && !seen_end) ||
+    if (((rounds & 1023) == 0 && !(seen_end & 2)) ||
         ((rounds & 15) == 0 && __any(L.stage == ST_JOB && (pending < 0 || pending / jpd >= seen_produced)))) {
       long long v = 0;
       int e = 0;

@@ -26,7 +26,7 @@ DDX_FN bool is_ns(int h) { return (h & 1) == 0; }
 struct Top {
   u64 lens;  // lengths() of the hands
   u64 hold;  // holder of the k-th highest card of suit s in nibble 3s+k; 0xF = none
-  u32 c0, c1, c2;  // k-th highest card (k = 0, 1, 2) of suit s in byte s
+  u32 c0, c1, c2;  // byte s of c0 / c1 / c2: the highest / second / third card left in suit s
 };
 
 DDX_FN int tlen(const Top& t, int h, int s) { return len_of(t.lens, h, s); }
@@ -276,8 +276,8 @@ DDX_FN bool has_void_entry(const Top& t, int trump, int a, int b) {
   return any;
 }
 
-// DDS QuickTricks for the side on lead (hand = the leader). `cutoff` = tricks the leading
-// side needs, `cutoff_other` = tricks the other side needs. Returns the quick tricks;
+// Quick-trick bound of DDS for the leader's side (`hand` leads). The leader's side is short
+// of `cutoff` tricks, its opponents of `cutoff_other`. Returns the quick tricks;
 // decided = true when that settles the node: then a positive count means the leading side
 // makes it, 0 that it fails (reference dd_quick_tricks, src/dd_bounds.h:245-438).
 DDX_FN int quick_tricks(const u64* h, const Top& t, int trump, int hand, int cutoff,
@@ -414,8 +414,8 @@ DDX_FN int quick_tricks(const u64* h, const Top& t, int trump, int hand, int cut
     suit = qt_next_suit(suit, trump);
   }
 
-  // No quick trick and no trump winner: the opponents win the first trick with a top card
-  // of any suit the leader may lead.
+  // Nothing cashes and trumps give no winner: whatever suit is led, a defender's top card
+  // takes the trick.
   if (qt == 0 && (trump < 0 || win_h(t, trump) < 0)) {
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -426,8 +426,8 @@ DDX_FN int quick_tricks(const u64* h, const Top& t, int trump, int hand, int cut
   return qt;
 }
 
-// DDS QuickTricksSecondHand: can the side of `hand` (second to play, after the lead `lead`)
-// win this trick and, in notrump, enough more for `cutoff` tricks? `t` is built from the
+// Second-hand quick tricks of DDS: after the opening card `lead`, does the partnership of the
+// second player `hand` take this trick and, in notrump, reach `cutoff` tricks from there? `t` is built from the
 // trick-start hands, `h` are the hands now (reference dd_quick_tricks_2nd,
 // src/dd_bounds.h:442-479).
 DDX_FN bool quick_tricks_2nd(const u64* h, const Top& t, int trump, int hand, int lead, int cutoff,
@@ -717,7 +717,7 @@ DDX_FN int lead_weight(const u64* h, const Top& t, int trump, int left, int lead
   return w;
 }
 
-// DDS rank scale: 2..14, 0 = no card.
+// Ranks as DDS numbers them (deuce = 2 .. ace = 14); an empty mask gives 0.
 DDX_FN int rank_of(int card) { return (card & 15) + 2; }
 DDX_FN int hi_rank(u64 m) { return m ? (msb(m) & 15) + 2 : 0; }
 DDX_FN int lo_rank(u64 m) { return m ? (lsb(m) & 15) + 2 : 0; }
@@ -726,7 +726,7 @@ DDX_FN int lo_rank(u64 m) { return m ? (lsb(m) & 15) + 2 : 0; }
 // of each suit when the trick started (reference DDFollow / dd_follow_init,
 // src/dd_bounds.h:734-763).
 struct Follow {
-  u64 allt;  // all cards at trick start
+  u64 allt;  // every card still out when the trick began
   u64 all;   // all cards now
   u32 top2;  // holder of the top / second card of suit s at trick start: nibbles 2s, 2s+1
   int lead, ls, lead0, high1, high2, move1, move2;
@@ -775,8 +775,8 @@ DDX_FN void follow_init(Follow& f, const u64* h, int trump, int leader, u32 tric
   f.move2 = w2 == 2 ? c2 : cw1;
 }
 
-// DDS weights for the 2nd, 3rd and 4th hand (weight_alloc_*1/2/3, without the
-// rank_forces_ace / get_top_number refinements); n = cards already played in the trick
+// Move weights of DDS for the players after the lead (its weight_alloc_*1/2/3 rules; the
+// rank_forces_ace and get_top_number refinements are not applied); n = cards already played in the trick
 // (reference dd_follow_weight, src/dd_bounds.h:767-949).
 DDX_FN int follow_weight(const Follow& f, const u64* h, int trump, int n, int seat, int card) {
   const int lead = f.lead, ls = f.ls;
