@@ -18,50 +18,21 @@
 // return the exact trick count; seeds only change how many searches it takes.
 //
 // Stages: JOB (take a job), ROOT (start a lead), ENTER (enter a node), NEXT (play the next
-// move of the top frame), RET (a child returned), EXIT, and UNIT (a helper subproblem --
-// a position with a trick window, see dds_gpu_amd/csrc/ddx_help.hpp -- has its value in lo).
+// move of the top frame), RET (a child returned), EXIT; with splitting (split.hpp) also WAIT
+// (the top frame's untried moves were published as tasks; wait for their join) and HELP (no
+// job is left: take published tasks).
 #pragma once
 #include "search.hpp"
+#include "split.hpp"
 
 #if DDX_VERIFY
 #include <stdio.h>
 #endif
 
-// DDX_HELP = 1 compiles the hooks for tail helpers (dds_gpu_amd/csrc/ddx_help.hpp, which sets
-// it); without it they fold away and the machine is exactly the plain one.
-#ifndef DDX_HELP
-#define DDX_HELP 0
-#endif
-
 namespace ddx {
 
-enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_UNIT, ST_COUNT };
+enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_WAIT, ST_HELP, ST_COUNT };
 
-// First-writer-wins flag and counter for answers that helpers may also produce (device
-// atomics; a host build is single-threaded per flag).
-DDX_FN bool claim_flag(u32* f) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return atomicCAS(f, 0u, 1u) == 0u;
-#else
-  if (*f) return false;
-  *f = 1;
-  return true;
-#endif
-}
-DDX_FN void count_one(unsigned long long* c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  atomicAdd(c, 1ull);
-#else
-  ++*c;
-#endif
-}
-DDX_FN bool flag_set(const u32* f) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-#else
-  return *f != 0;
-#endif
-}
 enum { SEED_PLAIN = 0, SEED_STEP = 1, SEED_CERT = 2, SEED_CERTP = 3 };
 
 // The batch a lane works on. Deal d's hands are hands[4d..4d+3]; its results go to
@@ -74,12 +45,10 @@ struct Batch {
   long long jobs;        // deals * jobs_per_deal
   int jobs_per_deal;     // 5 (per strain) or 20 (per lead)
   int seed_mode;
-#if DDX_HELP
-  // With helpers: one flag per answer (same layout as out); the first to finish a lead -- its
-  // own lane or the helpers -- claims it, writes out and counts it in *leads_done.
-  u32* lead_done = nullptr;
-  unsigned long long* leads_done = nullptr;
-#endif
+  // Splitting of long searches once the job queue is empty (split.hpp); off while
+  // sp.tasks is null, and then every split path folds away at compile time in kernels and
+  // host code that leave it so.
+  SplitView sp = {};
 };
 
 struct Lane {
@@ -95,8 +64,16 @@ struct Lane {
   int strain, decl, last_decl, prev_ns, last_true;
   int exit_at;  // where the last enter() decided (profiling)
   u32 jobs_done;
-  int rec;     // helper subproblem being solved, or -1 for the batch's own leads
   u32 serial;  // null-window searches started (identifies the current search)
+  // Splitting (split.hpp):
+  long long job;     // batch job of the current work (a task inherits its owner's)
+  long long tt_job;  // job whose positions the TT's current generation holds
+  long long ticket;  // task slot this idle lane waits for, or -1
+  int task;          // task being solved, or -1 for a job of the batch
+  int task_join;     // that task's join
+  int wait_join;     // join the top frame waits for (stage WAIT)
+  int may_split;     // lanes were waiting for work at the last look (refreshed by the caller)
+  u32 splits;        // nodes this lane split
 };
 
 DDX_FN void lane_init(Lane& L, TTView tt) {
@@ -118,8 +95,11 @@ DDX_FN void lane_init(Lane& L, TTView tt) {
   L.target = L.lo = L.hi = L.searches = 0;
   L.exit_at = 0;
   L.jobs_done = 0;
-  L.rec = -1;
   L.serial = 0;
+  L.job = L.tt_job = L.ticket = -1;
+  L.task = L.task_join = L.wait_join = -1;
+  L.may_split = 0;
+  L.splits = 0;
 }
 
 DDX_FN void start_search(Lane& L) {
@@ -147,30 +127,13 @@ DDX_FN void lane_load_job(Lane& L, const Batch& B, long long job) {
   L.last_decl = jpd == 5 ? 3 : L.decl;
   L.p.trump = strain_trump(L.strain);
   next_generation<X>(L.c);
+  L.job = L.tt_job = job;
   L.prev_ns = 7;
   L.stage = ST_ROOT;
 }
 
-// Moves on to the job's next lead, or to the next job.
-DDX_FN void lane_next_lead(Lane& L) {
-  L.decl++;
-  if (L.decl <= L.last_decl) {
-    L.stage = ST_ROOT;
-    return;
-  }
-  L.jobs_done++;
-  L.stage = ST_JOB;
-}
-
-// ROOT: first target of this lead's search sequence (a lead that helpers already answered is
-// skipped).
+// ROOT: first target of this lead's search sequence.
 DDX_FN void lane_root(Lane& L, const Batch& B) {
-#if DDX_HELP
-  if (B.lead_done && flag_set(B.lead_done + 20 * L.deal + 4 * L.strain + L.decl)) {
-    lane_next_lead(L);
-    return;
-  }
-#endif
   const int left = popc(L.p.hand[0]);
   L.lo = 0;
   L.searches = 0;
@@ -192,38 +155,26 @@ DDX_FN void lane_root(Lane& L, const Batch& B) {
 // The root search finished with L.v: narrow [lo, hi]; pick the next target, or record the
 // answer and move to the next lead or job (reference dd_wave_node_done, src/dd_wave.h:42-65).
 DDX_FN void lane_search_done(Lane& L, const Batch& B) {
+  if (B.sp.tasks && L.task >= 0) {  // a task: its value goes to the owner's join
+    const SplitTask& t = B.sp.tasks[L.task];
+    split_report(B.sp, L.task_join, L.v, t.goal, L.r | t.extra, t.card);
+    L.task = L.task_join = -1;
+    L.stage = ST_HELP;
+    return;
+  }
   L.searches++;
   if (L.v) L.lo = L.target;
   else L.hi = L.target - 1;
   if (L.lo < L.hi) {
-    const bool sub = DDX_HELP && L.rec >= 0;
-    const bool adjacent =
-        !sub && (B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 3));
+    const bool adjacent = B.seed_mode == SEED_STEP || (B.seed_mode >= SEED_CERT && L.searches < 3);
     if (adjacent) L.target = L.v ? L.lo + 1 : L.hi;
     else L.target = (L.lo + L.hi + 1) / 2;
-    if (sub) {  // a subproblem: the search unwound to its root position; search again
-      L.depth = 0;
-      L.serial++;
-      L.stage = ST_ENTER;
-    } else {
-      start_search(L);
-    }
-    return;
-  }
-  if (DDX_HELP && L.rec >= 0) {
-    L.stage = ST_UNIT;
+    start_search(L);
     return;
   }
   const int ns = L.lo;
   const long long idx = 20 * L.deal + 4 * L.strain + L.decl;
-#if DDX_HELP
-  if (!B.lead_done || claim_flag(B.lead_done + idx)) {
-    B.out[idx] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
-    if (B.leads_done) count_one(B.leads_done);
-  }
-#else
   B.out[idx] = (uint8_t)((L.decl & 1) == 0 ? ns : 13 - ns);
-#endif
   L.decl++;
   if (L.decl <= L.last_decl) {
     L.prev_ns = ns < 1 ? 1 : ns;
@@ -280,9 +231,76 @@ DDX_FN void lane_next(Lane& L, const Batch& B, Stack& st) {
   lane_node_done<X>(L, B, st);
 }
 
+// Publishes the top frame's untried moves as tasks and sends the lane to WAIT (split.hpp),
+// when lanes are waiting for work, the children searched so far took at least 16 * min_nodes16
+// nodes (so the rest are likely big too), the node has at least min_left tricks to go and at
+// least min_moves moves are left. Called at RET after a child that did not cut off, with L.p the
+// frame's position. A move is left out when the small-card rule would skip it now: that test
+// only uses children already searched (published moves do not count as tried, since their
+// relevant cards are not known yet). Returns whether it split.
+DDX_FN bool lane_split(Lane& L, const Batch& B) {
+  const SplitView& s = B.sp;
+  Frame& f = L.top;
+  if (L.p.left < s.min_left || f.n - f.i < s.min_moves) return false;
+  if ((((u32)(L.c.nodes >> 4) - f.nodes0) & 0xFFFFFFu) < s.min_nodes16) return false;
+  int k = 0;
+  for (int i = f.i; i < f.n; i++) k += small_skip(f, move_at(f, i)) ? 0 : 1;
+  if (k < s.min_moves || !split_starving(s)) return false;
+  const long long j = (long long)sp_add(s.ctr + 2, 1);
+  if (j >= s.join_cap) return false;
+  const long long base = (long long)sp_add(s.ctr + 0, (unsigned long long)k);
+  if (base + k > s.task_cap) {
+    // Out of slots: lanes may hold tickets for the reserved ones, so mark them skipped.
+    for (long long q = base; q < s.task_cap && q < base + k; q++) sp_store_rel(&s.tasks[q].state, SLOT_SKIP);
+    return false;
+  }
+  SplitJoin& J = s.joins[j];
+  J.acc = 0;
+  J.cut_rel = 0;
+  J.cut_card = -1;
+  J.pending = k;
+  J.parent = L.task >= 0 ? L.task_join : -1;
+  J.state = JOIN_OPEN;
+  const int8_t goal = (int8_t)((f.seat & 1) == 0 ? 1 : 0);
+  int q = 0;
+  for (int i = f.i; i < f.n; i++) {
+    const int card = move_at(f, i);
+    if (small_skip(f, card)) continue;
+    Pos c = L.p;
+    Frame g = f;
+    play(c, g, card);
+    u64 extra = 0;
+    Pos back = c;
+    unplay(back, g, card, extra);
+    SplitTask& t = s.tasks[base + q];
+#pragma unroll
+    for (int h = 0; h < 4; h++) t.hand[h] = c.hand[h];
+    t.extra = extra;
+    t.trick = c.trick;
+    t.join = (int)j;
+    t.job = (int)L.job;
+    t.trump = (int8_t)c.trump;
+    t.leader = (int8_t)c.leader;
+    t.nplayed = (int8_t)c.nplayed;
+    t.ns_won = (int8_t)c.ns_won;
+    t.left = (int8_t)c.left;
+    t.target = (int8_t)L.target;
+    t.card = (int8_t)card;
+    t.goal = goal;
+    sp_store_rel(&t.state, SLOT_READY);  // publishes the task and, before it, the join
+    q++;
+  }
+  f.i = f.n;
+  L.wait_join = (int)j;
+  L.stage = ST_WAIT;
+  L.splits++;
+  sp_add(s.ctr + 3, 1);
+  return true;
+}
+
 // RET: the child of the top frame returned (L.v, L.r): undo its move; a result that reaches
 // the mover's goal cuts off, else collect its relevant cards and try the next move
-// (reference dd_search child_done, src/dd.h:840-854).
+// (reference dd_search child_done, src/dd.h:840-854), or split the rest off (split.hpp).
 template <class X = SeqOps, class Stack>
 DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
   Frame& f = L.top;
@@ -299,6 +317,85 @@ DDX_FN void lane_ret(Lane& L, const Batch& B, Stack& st) {
   f.acc |= L.r;
   f.i++;
   L.stage = ST_NEXT;
+  if (B.sp.tasks && L.may_split) lane_split(L, B);
+}
+
+// WAIT: the top frame's remaining moves are tasks. A child that cut off decides the frame
+// (best move and relevant cards are that child's); when every task has reported without a
+// cutoff, the frame continues as if it had tried them itself (their relevant cards added), and
+// NEXT finds no move left.
+template <class X = SeqOps, class Stack>
+DDX_FN void lane_wait(Lane& L, const Batch& B, Stack& st) {
+  SplitJoin& J = B.sp.joins[L.wait_join];
+  const u32 state = sp_load_acq(&J.state);
+  Frame& f = L.top;
+  if (state == JOIN_CUT) {
+    f.result = (f.seat & 1) == 0 ? 1 : 0;
+    f.acc = J.cut_rel;
+    f.best = J.cut_card;
+    L.wait_join = -1;
+    L.v = f.result;
+    L.r = finish<X>(L.c, L.p, f);
+    lane_node_done<X>(L, B, st);
+    return;
+  }
+  if (state == JOIN_OPEN && sp_load_acq(&J.pending) == 0) {
+    f.acc |= sp_load64(&J.acc);
+    L.wait_join = -1;
+    L.stage = ST_NEXT;
+  }
+}
+
+// Drops the lane's task (its join chain was cut off): the stack is abandoned, and a join the
+// lane waited for is left to its own tasks, which find the same cut above them.
+DDX_FN void lane_drop_task(Lane& L) {
+  L.task = L.task_join = L.wait_join = -1;
+  L.depth = 0;
+  L.stage = ST_HELP;
+}
+
+// Whether the lane's task lies under a join that was cut off.
+DDX_FN bool lane_task_moot(const Lane& L, const Batch& B) {
+  return B.sp.tasks && L.task >= 0 && split_chain_cut(B.sp, L.task_join);
+}
+
+// HELP: an idle lane with ticket L.ticket (handed out by the caller) takes the task in that
+// slot once it is written; a skipped slot or a task already moot returns the ticket. Returns
+// true when the lane took a task (stage ENTER).
+template <class X = SeqOps>
+DDX_FN bool lane_help(Lane& L, const Batch& B) {
+  if (L.ticket < 0 || L.ticket >= B.sp.task_cap) return false;  // beyond the pool: stays idle
+  SplitTask& t = B.sp.tasks[L.ticket];
+  const u32 state = sp_load_acq(&t.state);
+  if (state == SLOT_EMPTY) return false;
+  const long long slot = L.ticket;
+  L.ticket = -1;
+  if (state == SLOT_SKIP) return false;
+  if (split_chain_cut(B.sp, t.join)) {
+    sp_add(B.sp.ctr + 5, 1);
+    return false;
+  }
+#pragma unroll
+  for (int h = 0; h < 4; h++) L.p.hand[h] = t.hand[h];
+  L.p.trick = t.trick;
+  L.p.trump = t.trump;
+  L.p.leader = t.leader;
+  L.p.nplayed = t.nplayed;
+  L.p.ns_won = t.ns_won;
+  L.p.left = t.left;
+  L.target = t.target;
+  L.task = (int)slot;
+  L.task_join = t.join;
+  L.job = t.job;
+  if (L.job != L.tt_job) {  // positions of another job: a fresh TT generation
+    next_generation<X>(L.c);
+    L.tt_job = L.job;
+  }
+  L.depth = 0;
+  L.serial++;
+  L.stage = ST_ENTER;
+  sp_add(B.sp.ctr + 4, 1);
+  return true;
 }
 
 // One stage of the machine. `take` returns the next job index (>= jobs: none left).
@@ -308,7 +405,7 @@ DDX_FN void lane_step(Lane& L, const Batch& B, Stack& st, Take take) {
     case ST_JOB: {
       long long job = take();
       if (job >= B.jobs) {
-        L.stage = ST_EXIT;
+        L.stage = B.sp.tasks ? ST_HELP : ST_EXIT;
         return;
       }
       lane_load_job<X>(L, B, job);

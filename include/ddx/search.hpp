@@ -62,6 +62,7 @@ struct Frame {
   u32 mv2;   // moves 10..12
   u32 save;  // the trick this ply completed (4 cards), for undo
   u32 low_tried;  // per suit (nibble s): 1 + rank of the lowest card tried, 0 = none
+  u32 nodes0;     // the context's node count / 16 when the frame was made, modulo 2^24
   int n, i, best, result, seat, need, lb, ub;
 };
 struct alignas(16) PackedFrame {
@@ -69,7 +70,7 @@ struct alignas(16) PackedFrame {
   u32 mv2;  // moves 10..12 (bits 0..17) | n (18..21) | i (22..25) | seat (26..27) | result (28)
   u32 save;
   u32 meta;  // low_tried (0..15) | best + 1 (16..22) | need (23..26)
-  u32 bounds;  // lb (0..3) | ub (4..7)
+  u32 bounds;  // lb (0..3) | ub (4..7) | nodes0 (8..31)
 };
 
 DDX_FN PackedFrame pack_frame(const Frame& f) {
@@ -79,7 +80,7 @@ DDX_FN PackedFrame pack_frame(const Frame& f) {
   q.mv2 = f.mv2 | ((u32)f.n << 18) | ((u32)f.i << 22) | ((u32)f.seat << 26) | ((u32)f.result << 28);
   q.save = f.save;
   q.meta = f.low_tried | ((u32)(f.best + 1) << 16) | ((u32)f.need << 23);
-  q.bounds = (u32)f.lb | ((u32)f.ub << 4);
+  q.bounds = (u32)f.lb | ((u32)f.ub << 4) | (f.nodes0 << 8);
   return q;
 }
 DDX_FN Frame unpack_frame(const PackedFrame& q) {
@@ -97,6 +98,7 @@ DDX_FN Frame unpack_frame(const PackedFrame& q) {
   f.need = (int)((q.meta >> 23) & 15);
   f.lb = (int)(q.bounds & 15);
   f.ub = (int)((q.bounds >> 4) & 15);
+  f.nodes0 = q.bounds >> 8;
   return f;
 }
 DDX_FN int move_at(const Frame& f, int i) {
@@ -321,6 +323,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   f.best = -1;
   f.acc = 0;
   f.low_tried = 0;
+  f.nodes0 = (u32)(c.nodes >> 4) & 0xFFFFFFu;
   exit_at = EX_MOVES;
   return -1;
 }
@@ -356,6 +359,7 @@ DDX_FN int enter_follow(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, in
   f.best = -1;
   f.acc = 0;
   f.low_tried = 0;
+  f.nodes0 = (u32)(c.nodes >> 4) & 0xFFFFFFu;
   exit_at = EX_MOVES;
   return -1;
 }
@@ -435,19 +439,27 @@ DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
   return acc;
 }
 
+// Whether `card` is a small card f need not try: below every relevant card of its suit
+// collected so far, with a small card of that suit already tried.
+DDX_FN bool small_skip(const Frame& f, int card) {
+  const int s = card >> 4, r = card & 15;
+  const u64 rs = f.acc & suit_mask(s);
+  const int cut = rs ? (lsb(rs) & 15) : 16;  // ranks below `cut` are small
+  const int lt = (int)((f.low_tried >> (4 * s)) & 15);  // 1 + lowest tried rank
+  return !DDX_NO_SMALL && r < cut && lt != 0 && lt - 1 < cut;
+}
+
 // The next move of f to try, skipping small cards (below every relevant card of their suit)
 // once one small card of that suit was tried; -1 when none is left.
 DDX_FN int next_move(Frame& f) {
   while (f.i < f.n) {
     const int card = move_at(f, f.i);
-    const int s = card >> 4, r = card & 15;
-    const u64 rs = f.acc & suit_mask(s);
-    const int cut = rs ? (lsb(rs) & 15) : 16;  // ranks below `cut` are small
-    const int lt = (int)((f.low_tried >> (4 * s)) & 15);  // 1 + lowest tried rank
-    if (!DDX_NO_SMALL && r < cut && lt != 0 && lt - 1 < cut) {
+    if (small_skip(f, card)) {
       f.i++;
       continue;
     }
+    const int s = card >> 4, r = card & 15;
+    const int lt = (int)((f.low_tried >> (4 * s)) & 15);
     if (lt == 0 || r + 1 < lt) f.low_tried = (f.low_tried & ~(15u << (4 * s))) | ((u32)(r + 1) << (4 * s));
     return card;
   }
