@@ -57,7 +57,13 @@ struct KernelArgs {
   volatile int* stop;            // host-mapped; nonzero = stop
   unsigned long long* wave_prog;  // host-mapped, 2 per wave: finished jobs, nodes up to them
   unsigned long long* prof;      // rounds [0..7], lanes [8..15], exits [16..22], cycles [24..31]
+  SplitView sp;                  // task pool of the split kernels (ddx/split.hpp)
 };
+
+// Lanes of a wave that may help with published tasks (lanes 0, 16, 32, 48): tasks land on
+// different waves -- a wave runs its lanes' searches one branch at a time, so four helpers in
+// one wave share its issue slots, and up to 64 would share them sixty-four ways.
+constexpr int kHelperStride = 16;
 
 // Publishes the wave's finished jobs and the nodes its searches spent up to them to its two
 // slots of host memory (plain stores by one lane), so the host reads progress without any GPU
@@ -78,7 +84,40 @@ __device__ __forceinline__ void publish_wave_progress(unsigned long long* wave_p
   }
 }
 
-template <int SCHED, bool PROF, int TEAM, bool LDS = false>
+// The split kernels' part of a round, before the cheap stages:
+//   * every 64 rounds: whether lanes wait for work and at most max_open_jobs jobs are unfinished
+//     (splits are tried only then: while many jobs run, a split would idle its owner with
+//     little to gain), and whether every job of the batch is done (lanes without one of them
+//     left then exit);
+//   * every 256 rounds: a lane whose task lies under a cut-off join drops it;
+//   * a lane waiting for its join checks it;
+//   * idle lanes -- at most four per wave -- draw tickets (one atomic add per wave) and take the
+//     task in their slot once it is written.
+// Nothing here waits inside the round: a lane that cannot go on retries next round.
+template <class Stack>
+__device__ __forceinline__ void split_round(Lane& L, const Batch& B, Stack& st, const KernelArgs& a,
+                                            unsigned rounds) {
+  if ((rounds & 63) == 0) {
+    const unsigned long long done = sp_load(a.counters + 5);
+    L.may_split = split_starving(B.sp) && (unsigned long long)a.jobs - done <= (unsigned long long)B.sp.max_open_jobs;
+    const bool all_done = done >= (unsigned long long)a.jobs;
+    if (all_done && (L.stage == ST_HELP || L.task >= 0)) L.stage = ST_EXIT;
+  }
+  if ((rounds & 255) == 0 && L.stage != ST_EXIT && lane_task_moot(L, B)) lane_drop_task(L);
+  if (L.stage == ST_WAIT) lane_wait<SeqOps>(L, B, st);
+  const bool want = L.stage == ST_HELP && L.ticket < 0 && (__lane_id() % kHelperStride) == 0;
+  const unsigned long long m = __ballot(want);
+  if (m) {
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)__lane_id() == leader) base = sp_add(B.sp.ctr + 1, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (want) L.ticket = (long long)(base + __popcll(m & ((1ull << __lane_id()) - 1)));
+  }
+  if (L.stage == ST_HELP) lane_help<SeqOps>(L, B);
+}
+
+template <int SCHED, bool PROF, int TEAM, bool LDS = false, bool SPLIT = false>
 __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   using TM = TeamOf<TEAM>;
   using X = typename std::conditional<LDS, LdsOps, typename TM::Ops>::type;
@@ -87,9 +126,24 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   Lane L;
   GpuStack st;
   lane_attach(a.lanes, lane, L, st);
-  const Batch B{a.hands, a.out, a.seeds, a.jobs, a.jobs_per_deal, a.seed_mode};
+  // Without SPLIT the task pool is a compile-time null, and every split path folds away.
+  const Batch B{a.hands, a.out, a.seeds, a.jobs, a.jobs_per_deal, a.seed_mode, SPLIT ? a.sp : SplitView{}};
   unsigned long long* next = a.counters;
-  auto take = [&]() -> long long { return TM::take(next); };
+  // Each search's first job is assigned so that consecutive jobs go to different waves (job j
+  // to slot j / W of wave j % W, W waves): a batch with fewer jobs than searches then puts one
+  // or a few searches in each wave instead of filling the first waves, whose lanes would run
+  // their divergent searches one branch at a time. Later jobs come from the atomic counter,
+  // after the S statically assigned ones. Each job's search is the same wherever it runs.
+  const long long S = (long long)gridDim.x * blockDim.x / TEAM, per_wave = 64 / TEAM;
+  bool fresh = true;
+  auto take = [&]() -> long long {
+    if (fresh) {
+      fresh = false;
+      const long long j = (lane % per_wave) * (S / per_wave) + lane / per_wave;
+      if (j < a.jobs) return j;
+    }
+    return S + TM::take(next);
+  };
   // Profile (PROF only; wave-uniform counters kept by every lane, lane 0 reports):
   // kinds 0..5 = stage rounds of the PRIORITY schedule, 6 = cheap chains (with the follow
   // nodes' ENTER under SPLIT), 2 = (trick-start) ENTER in the other schedules.
@@ -116,14 +170,17 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
   // that a child's return and the next move, or a search's end and the next one's start, take
   // one pass rather than one per stage. Under SPLIT a pass also enters follow nodes (2nd-4th
   // hand), so the chain runs until every lane waits at a trick start (or has no work).
+  // With SPLIT a lane also leaves the chain when it waits for a join (WAIT) or has no job left
+  // (HELP); the round loop serves both.
   auto cheap_chain = [&](unsigned& passes) {
     while (L.stage != ST_EXIT && (L.stage != ST_ENTER || (SCHED == DDX_SCHED_SPLIT && L.p.nplayed != 0))) {
+      if (SPLIT && (L.stage == ST_WAIT || L.stage == ST_HELP)) break;
       if (PROF) passes++;
       if (L.stage == ST_RET) lane_ret<X>(L, B, st);
       if (L.stage == ST_NEXT) lane_next<X>(L, B, st);
       if (L.stage == ST_JOB) {
         const long long job = take();
-        if (job >= a.jobs) L.stage = ST_EXIT;
+        if (job >= a.jobs) L.stage = SPLIT ? ST_HELP : ST_EXIT;
         else lane_load_job<X>(L, B, job);
       }
       if (L.stage == ST_ROOT) lane_root(L, B);
@@ -146,6 +203,7 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
       reported = L.jobs_done;
       reported_nodes = L.c.nodes;
     }
+    if (SPLIT) split_round(L, B, st, a, rounds);
     if (SCHED == DDX_SCHED_PRIORITY) {
       if (__all(L.stage == ST_EXIT)) break;
       int pick;
@@ -180,6 +238,9 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
         p_cycles[6] += (unsigned long long)(clock64() - t0);
       }
       if (__all(L.stage == ST_EXIT)) break;
+      // A wave with no node to enter only polls (its lanes wait for joins or for work): sleep
+      // a little so that idle waves do not crowd the memory system with those polls.
+      if (SPLIT && !__any(L.stage == ST_ENTER)) __builtin_amdgcn_s_sleep(16);
       // The trick-start nodes (SPLIT), or every node (CHAIN), entered with the wave gathered.
       const bool run = L.stage == ST_ENTER;
       t0 = PROF ? clock64() : 0;
@@ -228,6 +289,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 template <int TEAM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void solve_kernel_w4(KernelArgs a) {
   solve_body<DDX_SCHED_SPLIT, false, TEAM>(a);
+}
+// One lane per search with splitting of the tail (ddx/split.hpp), at the compiler's allocation
+// or held to 3 waves per SIMD.
+__global__ __launch_bounds__(256) void solve_kernel_split(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, 1, false, true>(a);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void solve_kernel_split_w3(KernelArgs a) {
+  solve_body<DDX_SCHED_SPLIT, false, 1, false, true>(a);
 }
 // One lane per search with the LDS cache of tag lines (ddx_lds.hpp), at the compiler's
 // allocation or held to 3 waves per SIMD.
@@ -325,6 +394,11 @@ struct ddx_gpu {
   hipStream_t own = nullptr;     // non-blocking stream for calls that pass none
   hipStream_t side = nullptr;    // non-blocking stream for progress reads
   hipGraphExec_t gexec = nullptr;  // captured batch (config.graph)
+  // Task pool of the split kernels (config.split_min_nodes): cleared at every launch.
+  SplitTask* sp_tasks = nullptr;
+  SplitJoin* sp_joins = nullptr;
+  unsigned long long* sp_ctr = nullptr;
+  long long sp_cap = 0;
   int64_t g_n = -1;
   const void* g_hands = nullptr;
   const void* g_out = nullptr;
@@ -349,6 +423,7 @@ extern "C" void ddx_gpu_default_config(ddx_gpu_config* cfg) {
   cfg->graph = 0;
   cfg->waves_per_simd = 0;
   cfg->lds_lines = 0;
+  cfg->split_min_nodes = 0;
 }
 
 static void free_all(ddx_gpu* h) {
@@ -365,6 +440,9 @@ static void free_all(ddx_gpu* h) {
   if (h->own) (void)hipStreamDestroy(h->own);
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->sp_tasks) (void)hipFree(h->sp_tasks);
+  if (h->sp_joins) (void)hipFree(h->sp_joins);
+  if (h->sp_ctr) (void)hipFree(h->sp_ctr);
 }
 
 
@@ -380,9 +458,13 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
       (cfg.waves_per_simd != 0 &&
        (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile)) ||
       (cfg.lds_lines != 0 && (cfg.lds_lines != 3 || cfg.team != 1 || cfg.schedule != DDX_SCHED_SPLIT ||
-                              cfg.profile || cfg.waves_per_simd == 4))) {
+                              cfg.profile || cfg.waves_per_simd == 4)) ||
+      cfg.split_min_nodes < 0 ||
+      (cfg.split_min_nodes > 0 && (cfg.team != 1 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile ||
+                                   cfg.lds_lines || cfg.waves_per_simd == 4))) {
     ddx_set_error("ddx_gpu_open: bad config (lanes %lld must be a multiple of block %d and of 64, "
-                  "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20, team 1 or 16)",
+                  "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20, team 1 or 16; splitting needs one "
+                  "lane per search, the split schedule, no profile, no LDS cache, 2 or 3 waves per SIMD)",
                   (long long)cfg.lanes, cfg.block, cfg.tt_log2);
     return DDX_E_ARG;
   }
@@ -407,6 +489,15 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
     return rc;
   }
   if (hipMalloc(&h->counters, 16 * 8) != hipSuccess) return fail("counters");
+  if (cfg.split_min_nodes > 0) {
+    // Eight task slots per lane: a split publishes at most 12, and nodes are split only while
+    // lanes wait for work.
+    h->sp_cap = cfg.lanes * 8;
+    if (hipMalloc(&h->sp_tasks, (size_t)h->sp_cap * sizeof(SplitTask)) != hipSuccess ||
+        hipMalloc(&h->sp_joins, (size_t)h->sp_cap * sizeof(SplitJoin)) != hipSuccess ||
+        hipMalloc(&h->sp_ctr, 8 * 8) != hipSuccess)
+      return fail("split task pool");
+  }
   if (hipMalloc(&h->prof, 32 * 8) != hipSuccess) return fail("profile counters");
   if (hipHostMalloc((void**)&h->h_stop, 64, hipHostMallocMapped) != hipSuccess) return fail("stop flag");
   *h->h_stop = 0;
@@ -420,7 +511,7 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   if (hipStreamCreateWithFlags(&h->own, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess)
     return fail("streams");
-  h->bytes = h->lanes.bytes;
+  h->bytes = h->lanes.bytes + (uint64_t)h->sp_cap * (sizeof(SplitTask) + sizeof(SplitJoin));
   if ((rc = launch_fill(h->prof, 32 * 8, 0, nullptr)) != DDX_OK || hipDeviceSynchronize() != hipSuccess) {
     free_all(h);
     delete h;
@@ -456,6 +547,10 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
   int rc;
   if ((rc = launch_fill(h->counters, 128, 0, s)) != DDX_OK) return rc;
   if ((rc = launch_fill(h->d_prog, (size_t)(h->cfg.lanes / 64) * 16, 0, s)) != DDX_OK) return rc;
+  if (h->sp_tasks) {  // every slot SLOT_EMPTY, the counters zero
+    if ((rc = launch_fill(h->sp_ctr, 8 * 8, 0, s)) != DDX_OK) return rc;
+    if ((rc = launch_fill(h->sp_tasks, (size_t)h->sp_cap * sizeof(SplitTask), 0, s)) != DDX_OK) return rc;
+  }
   if (n) {
     int grid = (int)((20 * n + 255) / 256 < 65536 ? (20 * n + 255) / 256 : 65536);
     hipLaunchKernelGGL(fill_bytes_kernel, dim3(grid), dim3(256), 0, s, a.out, (size_t)(20 * n),
@@ -466,6 +561,12 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
   const int grid = (int)(h->cfg.lanes / block);
   const bool P = h->cfg.profile != 0;
   const int W = h->cfg.waves_per_simd;
+  if (h->sp_tasks) {  // one lane per search, split schedule, no profile (checked at open)
+    if (W == 3) hipLaunchKernelGGL(solve_kernel_split_w3, dim3(grid), dim3(block), 0, s, a);
+    else hipLaunchKernelGGL(solve_kernel_split, dim3(grid), dim3(block), 0, s, a);
+    HIPCHECK(hipGetLastError());
+    return DDX_OK;
+  }
   if (h->cfg.lds_lines) {  // one lane per search, split schedule, no profile (checked at open)
     if (W == 3) hipLaunchKernelGGL(solve_kernel_lds_w3, dim3(grid), dim3(block), 0, s, a);
     else hipLaunchKernelGGL(solve_kernel_lds, dim3(grid), dim3(block), 0, s, a);
@@ -517,6 +618,15 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
   a.stop = h->d_stop;
   a.wave_prog = h->d_prog;
   a.prof = h->prof;
+  a.sp = SplitView{};
+  if (h->sp_tasks) {
+    a.sp.tasks = h->sp_tasks;
+    a.sp.joins = h->sp_joins;
+    a.sp.ctr = h->sp_ctr;
+    a.sp.task_cap = a.sp.join_cap = h->sp_cap;
+    a.sp.min_nodes16 = (u32)((h->cfg.split_min_nodes + 15) / 16);
+    a.sp.max_open_jobs = h->cfg.lanes / 64;  // a quarter of the helpers (four per wave)
+  }
   h->t0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(h->ev0, s));
   if (!h->cfg.graph) {
@@ -566,6 +676,13 @@ extern "C" int ddx_gpu_wait(ddx_gpu* h, ddx_stats* stats) {
     stats->tt_entry_reads = c[6];
     stats->tt_stores = c[7];
     stats->tt_line_cache_hits = c[9];
+    stats->splits = stats->tasks_taken = 0;
+    if (h->sp_ctr) {
+      unsigned long long sc[8];
+      HIPCHECK(hipMemcpy(sc, h->sp_ctr, sizeof sc, hipMemcpyDeviceToHost));
+      stats->splits = sc[3];
+      stats->tasks_taken = sc[4];
+    }
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h->t0).count();
   }
   if ((int64_t)c[4] < h->last_jobs) {

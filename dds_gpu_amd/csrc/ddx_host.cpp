@@ -292,6 +292,7 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
     stats->tt_entry_reads = ent_reads;
     stats->tt_stores = stores;
     stats->tt_line_cache_hits = 0;
+    stats->splits = stats->tasks_taken = 0;
     stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return DDX_OK;

@@ -217,9 +217,10 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   else ddx_gpu_default_config(&cfg);
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 256 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3 ||
-      (cfg.team != 1 && cfg.team != 16) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3)) {
+      (cfg.team != 1 && cfg.team != 16) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3) ||
+      cfg.split_min_nodes != 0) {
     ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, 20], "
-                  "jobs_per_deal 5 or 20)");
+                  "jobs_per_deal 5 or 20; the stream kernel does not split searches)");
     return DDX_E_ARG;
   }
   int ndev = 0;

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libddx.so")
 
 SEED_PLAIN, SEED_STEP, SEED_CERT, SEED_CERTP = 0, 1, 2, 3
 SCHED_PRIORITY, SCHED_CHAIN, SCHED_SPLIT = 0, 1, 2
-E_STOPPED = -5
+E_ARG, E_STOPPED = -1, -5
 
 _lib = None
 
@@ -22,7 +22,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_uint64), ("tt_probes", ctypes.c_uint64),
                 ("tt_hits", ctypes.c_uint64), ("jobs_done", ctypes.c_uint64),
                 ("seconds", ctypes.c_double), ("tt_entry_reads", ctypes.c_uint64),
-                ("tt_stores", ctypes.c_uint64), ("tt_line_cache_hits", ctypes.c_uint64)]
+                ("tt_stores", ctypes.c_uint64), ("tt_line_cache_hits", ctypes.c_uint64),
+                ("splits", ctypes.c_uint64), ("tasks_taken", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -33,7 +34,7 @@ class GpuConfig(ctypes.Structure):
                 ("jobs_per_deal", ctypes.c_int), ("schedule", ctypes.c_int),
                 ("seed_mode", ctypes.c_int), ("block", ctypes.c_int), ("profile", ctypes.c_int),
                 ("team", ctypes.c_int), ("graph", ctypes.c_int), ("waves_per_simd", ctypes.c_int),
-                ("lds_lines", ctypes.c_int)]
+                ("lds_lines", ctypes.c_int), ("split_min_nodes", ctypes.c_int)]
 
 
 class DdxError(RuntimeError):
@@ -192,7 +193,7 @@ class GpuSolver:
 
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=5, schedule=SCHED_CHAIN,
                  seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1, graph=False,
-                 waves_per_simd=0, lds_lines=0):
+                 waves_per_simd=0, lds_lines=0, split_min_nodes=0):
         cfg = GpuConfig()
         lib().ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
@@ -200,6 +201,7 @@ class GpuSolver:
         cfg.block, cfg.profile, cfg.team, cfg.graph = block, int(profile), team, int(graph)
         cfg.waves_per_simd = waves_per_simd
         cfg.lds_lines = lds_lines
+        cfg.split_min_nodes = split_min_nodes
         self.cfg = cfg
         self.h = ctypes.c_void_p()
         _check(lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))

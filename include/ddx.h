@@ -49,6 +49,8 @@ typedef struct ddx_stats {
   uint64_t tt_entry_reads; /* 16-byte TT entries read by probes */
   uint64_t tt_stores;    /* TT stores (each rewrites a 64-byte tag line and one entry) */
   uint64_t tt_line_cache_hits; /* tag lines served by the LDS line cache (config.lds_lines) */
+  uint64_t splits;       /* search nodes split over idle lanes (config.split_min_nodes) */
+  uint64_t tasks_taken;  /* published child searches that idle lanes took */
 } ddx_stats;
 
 /* Message of the last failure on this thread ("" if none). */
@@ -128,6 +130,12 @@ typedef struct ddx_gpu_config {
                            scratch); split schedule only, no profile */
   int lds_lines;        /* 0, or 3: each lane keeps its last 3 TT tag lines in LDS (write-through;
                            one lane per search, split schedule, no profile, waves_per_simd 0/3) */
+  int split_min_nodes;  /* 0: off. Otherwise, once the job queue is empty, a search node whose
+                           children searched so far took at least this many nodes publishes its
+                           untried moves as tasks for idle lanes (young-brothers-wait; exact).
+                           One lane per search, split schedule, no profile, waves_per_simd 0/3,
+                           no LDS cache. No counterpart in the reference, whose tail runs on the
+                           lanes that hold it (README.md:189-190). */
 } ddx_gpu_config;
 
 enum {

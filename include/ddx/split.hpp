@@ -74,6 +74,7 @@ struct SplitView {
   u32 min_nodes16 = 1024;  // split only nodes whose searched children took >= 16 * this many nodes
   int min_left = 3;        // ... with at least this many tricks left
   int min_moves = 2;       // ... and at least this many untried moves
+  long long max_open_jobs = 1ll << 62;  // splits start when at most this many jobs are unfinished
 };
 
 // Memory-order helpers: agent-scope atomics on the GPU; the host simulation runs its lanes on

@@ -57,7 +57,8 @@ extern "C" int ddx_split_sim(const uint64_t* hands, int64_t n, int lanes, int tt
       Lane& l = L[i];
       if (l.stage == ST_EXIT) continue;
       const u32 jobs0 = l.jobs_done;
-      if (split && (round & 63) == 0) l.may_split = split_starving(B.sp);
+      if (split && (round & 63) == 0)
+        l.may_split = split_starving(B.sp) && B.jobs - done <= B.sp.max_open_jobs;
       if (split && (round & 255) == 0 && lane_task_moot(l, B)) {
         lane_drop_task(l);
         abandoned++;
