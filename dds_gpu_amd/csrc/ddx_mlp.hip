@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <random>
 #include <vector>
@@ -324,22 +325,77 @@ static void gemm(hipStream_t s, const bf16* A, const bf16* B, float* C, int M, i
                      ldc, kchunk);
 }
 
-// Forward pass for the n deals staged in d_hands (labels when tables != null): Z3 = logits
+// Activations of one forward pass (rows padded to a multiple of 16). Z2 may share Z1's memory:
+// Z1 is dead once H1 is computed.
+struct Acts {
+  bf16* X;
+  float* Z1;
+  bf16* H1;
+  float* Z2;
+  bf16* H2;
+  float* Z3;
+  int* labels;
+};
+
+// Forward pass for n deals in d_hands on stream s (labels when tables != null): Z3 = logits
 // without the output bias.
-static void forward(ddx_mlp* h, const uint64_t* d_hands, const uint8_t* d_tables, long long n) {
+static void forward_into(const ddx_mlp* h, hipStream_t s, const Acts& a, const uint64_t* d_hands,
+                         const uint8_t* d_tables, long long n) {
   const long long S = n * 20, M = (S + 15) / 16 * 16;
-  hipLaunchKernelGGL(encode_kernel, dim3(blocks(S)), dim3(256), 0, h->s, d_hands, d_tables, n, h->X,
-                     d_tables ? h->labels : nullptr);
+  hipLaunchKernelGGL(encode_kernel, dim3(blocks(S)), dim3(256), 0, s, d_hands, d_tables, n, a.X,
+                     d_tables ? a.labels : nullptr);
   const bf16* W1 = h->w + OFF_W1;
   const bf16* W2 = h->w + OFF_W2;
   const bf16* W3 = h->w + OFF_W3;
-  gemm<false, true, false>(h->s, h->X, W1, h->Z1, (int)M, H, IN, IN, IN, H, IN);
-  hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks(M * H)), dim3(256), 0, h->s, h->Z1, h->p + OFF_B1,
-                     h->H1, M, H);
-  gemm<false, true, false>(h->s, h->H1, W2, h->Z2, (int)M, H, H, H, H, H, H);
-  hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks(M * H)), dim3(256), 0, h->s, h->Z2, h->p + OFF_B2,
-                     h->H2, M, H);
-  gemm<false, true, false>(h->s, h->H2, W3, h->Z3, (int)M, OUT, H, H, H, OUT, H);
+  gemm<false, true, false>(s, a.X, W1, a.Z1, (int)M, H, IN, IN, IN, H, IN);
+  hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks(M * H)), dim3(256), 0, s, a.Z1, h->p + OFF_B1, a.H1, M, H);
+  gemm<false, true, false>(s, a.H1, W2, a.Z2, (int)M, H, H, H, H, H, H);
+  hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks(M * H)), dim3(256), 0, s, a.Z2, h->p + OFF_B2, a.H2, M, H);
+  gemm<false, true, false>(s, a.H2, W3, a.Z3, (int)M, OUT, H, H, H, OUT, H);
+}
+
+static void forward(ddx_mlp* h, const uint64_t* d_hands, const uint8_t* d_tables, long long n) {
+  forward_into(h, h->s, Acts{h->X, h->Z1, h->H1, h->Z2, h->H2, h->Z3, h->labels}, d_hands, d_tables, n);
+}
+
+// ---- scratch of the allocation-free prediction ----------------------------------------------
+// Activations for at most kNoAllocDeals deals (20 samples each) in one file-scope device array:
+// X | Z (Z1, then Z2) | H1 | H2 | Z3, every part a multiple of 256 bytes. 13.1 MB, held by every
+// GPU the library runs on until the process ends. In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each GPU comes from hipGetSymbolAddress
+// (resolved once per device by ddx_mlp_create) and only that address is ever used.
+constexpr long long kNoAllocDeals = 256, kNoAllocRows = kNoAllocDeals * 20;
+constexpr size_t kNaX = 0, kNaZ = kNaX + (size_t)kNoAllocRows * IN * 2,
+                 kNaH1 = kNaZ + (size_t)kNoAllocRows * H * 4, kNaH2 = kNaH1 + (size_t)kNoAllocRows * H * 2,
+                 kNaZ3 = kNaH2 + (size_t)kNoAllocRows * H * 2, kNaBytes = kNaZ3 + (size_t)kNoAllocRows * OUT * 4;
+static_assert(kNaZ % 256 == 0 && kNaH1 % 256 == 0 && kNaH2 % 256 == 0 && kNaZ3 % 256 == 0, "scratch parts");
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices];  // device address of `scratch` per GPU
+
+// With `device` current: looks up the device address of `scratch` on it, once.
+static int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    ddx_set_error("device %d is beyond the %d devices the scratch table covers", device, kMaxDevices);
+    return DDX_E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DDX_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    ddx_set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    return DDX_E_HIP;
+  }
+  g_scratch_ptr[device] = (unsigned char*)p;
+  return DDX_OK;
+}
+
+static unsigned char* scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
 }
 
 // Gradients of the mean loss over the staged batch into h->g; loss/accuracy into stats.
@@ -384,6 +440,10 @@ extern "C" int ddx_mlp_create(int device, uint64_t seed, ddx_mlp** out) {
   ddx_mlp* h = new (std::nothrow) ddx_mlp;
   if (!h) return DDX_E_NOMEM;
   h->device = device;
+  if (int rc = resolve_scratch(device)) {
+    delete h;
+    return rc;
+  }
   HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
   HIPCHECK(hipMalloc(&h->p, NPARAM * 4));
   HIPCHECK(hipMalloc(&h->g, NPARAM * 4));
@@ -537,6 +597,43 @@ extern "C" int ddx_mlp_predict(ddx_mlp* h, const uint64_t* d_hands, int64_t n, u
   } else {
     HIPCHECK(hipStreamSynchronize(h->s));
   }
+  return DDX_OK;
+}
+
+// ddx_mlp_predict without any allocation: the activations live in the file-scope `scratch` (at
+// most kNoAllocDeals deals per call), every kernel runs on the caller's stream (NULL: the null
+// stream, and the call waits for the seeds). Calls that overlap on one GPU share that scratch
+// and race; the caller orders them.
+extern "C" int ddx_mlp_predict_noalloc(const ddx_mlp* h, const uint64_t* d_hands, int64_t n,
+                                       uint8_t* d_seeds, void* stream) {
+  if (!h || n < 0 || (n > 0 && (!d_hands || !d_seeds))) {
+    ddx_set_error("ddx_mlp_predict_noalloc: bad arguments");
+    return DDX_E_ARG;
+  }
+  if (n > kNoAllocDeals) {
+    ddx_set_error("ddx_mlp_predict_noalloc: %lld deals exceed its scratch of %lld deals", (long long)n,
+                  kNoAllocDeals);
+    return DDX_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(h->device));
+  unsigned char* base = scratch_of(h->device);
+  if (!base) {
+    ddx_set_error("ddx_mlp_predict_noalloc: no scratch on device %d", h->device);
+    return DDX_E_HIP;
+  }
+  if (n == 0) return DDX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  float* Z = (float*)(base + kNaZ);
+  const Acts a{(bf16*)(base + kNaX), Z, (bf16*)(base + kNaH1), Z, (bf16*)(base + kNaH2),
+               (float*)(base + kNaZ3), nullptr};
+  const long long S = n * 20, M = (S + 15) / 16 * 16;
+  if (M > S)  // padding rows of the input: zero (the scratch keeps the last call's contents)
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks((M - S) * IN / 2)), dim3(256), 0, s, (float*)(a.X + S * IN),
+                       (M - S) * IN / 2, 0.f);
+  forward_into(h, s, a, d_hands, nullptr, n);
+  hipLaunchKernelGGL(argmax_kernel, dim3(blocks(S)), dim3(256), 0, s, a.Z3, h->p + OFF_B3, S, d_seeds);
+  HIPCHECK(hipGetLastError());
+  if (!stream) HIPCHECK(hipStreamSynchronize(nullptr));
   return DDX_OK;
 }
 

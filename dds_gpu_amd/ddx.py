@@ -94,6 +94,7 @@ def lib():
         L.ddx_mlp_train.argtypes = [P, P, P, I64, I, I, ctypes.c_float, U64, P]
         L.ddx_mlp_predict.argtypes = [P, P, I64, P, P]
         L.ddx_mlp_predict_host.argtypes = [P, P, I64, P]
+        L.ddx_mlp_predict_noalloc.argtypes = [P, P, I64, P, P]
         L.ddx_mlp_save.argtypes = [P, ctypes.c_char_p]
         L.ddx_mlp_load.argtypes = [P, ctypes.c_char_p]
         _lib = L
@@ -386,6 +387,11 @@ class TrickNet:
     def predict_device(self, d_hands, n, d_seeds, stream=0):
         _check(lib().ddx_mlp_predict(self.h, ctypes.c_void_p(d_hands), n, ctypes.c_void_p(d_seeds),
                                      ctypes.c_void_p(stream) if stream else None))
+
+    def predict_device_noalloc(self, d_hands, n, d_seeds, stream=0):
+        """ddx_mlp_predict_noalloc: at most 256 deals, activations in the library's fixed scratch."""
+        _check(lib().ddx_mlp_predict_noalloc(self.h, ctypes.c_void_p(d_hands), n, ctypes.c_void_p(d_seeds),
+                                             ctypes.c_void_p(stream) if stream else None))
 
     def save(self, path):
         _check(lib().ddx_mlp_save(self.h, path.encode()))

@@ -214,6 +214,14 @@ int ddx_mlp_train(ddx_mlp* h, const uint64_t* hands, const uint8_t* tables, int6
 int ddx_mlp_predict(ddx_mlp* h, const uint64_t* d_hands, int64_t n, uint8_t* d_seeds, void* stream);
 /* Same with host buffers. */
 int ddx_mlp_predict_host(ddx_mlp* h, const uint64_t* hands, int64_t n, uint8_t* seeds);
+/* ddx_mlp_predict that allocates nothing, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops). No counterpart in the reference. The activations live in
+ * a fixed device scratch of the library (13.1 MB per GPU, held until the process ends), so n is
+ * at most 256 deals (DDX_E_ARG beyond). Kernels run on `stream` (NULL: the null stream, and the
+ * call returns when the seeds are written). Every call on one GPU shares that scratch: calls
+ * that overlap on a GPU race, and the caller orders them. */
+int ddx_mlp_predict_noalloc(const ddx_mlp* h, const uint64_t* d_hands, int64_t n, uint8_t* d_seeds,
+                            void* stream);
 int ddx_mlp_save(ddx_mlp* h, const char* path);
 int ddx_mlp_load(ddx_mlp* h, const char* path);
 
