@@ -21,6 +21,8 @@ Fixtures:
                    move-ordering helpers (layout in oracle/ref_harness.cpp).
   dds_4000.npy     rows 0..3999 of the reference fixture data/deals_20k.npy (Pgx DDS tables;
                    Apache-2.0, Pgx authors), kept as data.
+  mixed_endings.npz  300 random endings each of 5, 7, 9 and 11 cards per hand (seed 3000 + K),
+                   reference tables at TT_LOG2 10: the endings of the uneven (mixed) batch.
 """
 import argparse
 import os
@@ -38,6 +40,7 @@ sys.path.insert(0, ROOT)
 C1_SEED, C1_K, C1_N = 1, 8, 4096
 C2_SEED, FULL_N, FULL_CHUNK = 2026, 2000, 250
 FUNCS_N, FUNCS_SEED = 6000, 77
+MIXED_KS, MIXED_N, MIXED_SEED = (5, 7, 9, 11), 300, 3000
 
 
 def harness(ref, *args):
@@ -96,6 +99,20 @@ def make_full(ref):
                         seed=C2_SEED, chunk=FULL_CHUNK, tt_log2=10)
 
 
+def make_mixed(ref):
+    hands, ks = [], []
+    for k in MIXED_KS:
+        hands.append(deals(MIXED_SEED + k, 0, MIXED_N, k))
+        ks += [k] * MIXED_N
+    h = np.concatenate(hands)
+    chunks = [h[i:i + MIXED_N] for i in range(0, len(h), MIXED_N)]
+    with ThreadPoolExecutor(max_workers=len(chunks)) as ex:
+        res = list(ex.map(lambda c: ref_tables(ref, c, 10), chunks))
+    t = np.concatenate([r[0] for r in res])
+    np.savez_compressed(os.path.join(HERE, "mixed_endings.npz"), hands=h, tables=t, k=np.array(ks, np.int8),
+                        seed=MIXED_SEED, tt_log2=10)
+
+
 def make_funcs(ref):
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "f.bin")
@@ -115,7 +132,7 @@ def main():
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "REF=" + a.ref])
-    jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full}
+    jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full, "mixed": make_mixed}
     for name, fn in jobs.items():
         if a.only and name not in a.only.split(","):
             continue

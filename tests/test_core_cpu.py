@@ -137,3 +137,10 @@ def test_errors_are_reported(ddx):
         ddx.CpuContext(tt_log2=8)
     with pytest.raises(ddx.DdxError):
         ddx.cpu_solve(ddx.random_deals(1, 2, 5), tt_log2=8)
+
+
+def test_mixed_endings_fixture(ddx):
+    """Endings of 5, 7, 9 and 11 cards: the CPU machine gives the reference solver's tables."""
+    m = golden("mixed_endings.npz")
+    out, _ = ddx.cpu_solve(m["hands"], threads=8, tt_log2=10)
+    assert np.array_equal(out, m["tables"])

@@ -88,3 +88,14 @@ def test_pgx_decoding_matches_dds_fixture(oracle, ddx, tmp_path):
     q = str(tmp_path / "rt.npy")
     ddx.pgx_write(q, hands[:100], tables[:100])
     assert np.array_equal(np.load(q), a[:, :100, :])
+
+
+def test_mixed_fixture_deals_and_small_endings(oracle, ddx):
+    """The mixed-endings fixture: its deals are the seeded generator's, and the oracle agrees with
+    the reference solver's tables on its 5- and 7-card endings."""
+    m = golden("mixed_endings.npz")
+    for k in (5, 7, 9, 11):
+        rows = m["k"] == k
+        assert np.array_equal(m["hands"][rows], ddx.random_deals(3000 + k, int(rows.sum()), k))
+    small = np.isin(m["k"], (5, 7))
+    assert np.array_equal(oracle.tables(m["hands"][small]), m["tables"][small])
