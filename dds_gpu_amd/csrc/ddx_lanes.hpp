@@ -18,10 +18,23 @@ namespace ddx {
 
 constexpr int kStackDepth = 52;  // > the 48 expanded frames a full deal can need
 
+// Frame stacks are lane-major: a lane's frames are contiguous, so a push and the matching pop
+// hit the same line. DDX_FRAMES_DEPTH_MAJOR=1 builds the alternative layout -- frame i of every
+// lane contiguous, lanes at the same depth side by side -- for measuring one against the other.
+#ifndef DDX_FRAMES_DEPTH_MAJOR
+#define DDX_FRAMES_DEPTH_MAJOR 0
+#endif
+
 struct GpuStack {
   PackedFrame* base;
+#if DDX_FRAMES_DEPTH_MAJOR
+  long long stride;  // searches
+  __device__ __forceinline__ void store(int i, const Frame& f) { base[i * stride] = pack_frame(f); }
+  __device__ __forceinline__ Frame load(int i) const { return unpack_frame(base[i * stride]); }
+#else
   __device__ __forceinline__ void store(int i, const Frame& f) { base[i] = pack_frame(f); }
   __device__ __forceinline__ Frame load(int i) const { return unpack_frame(base[i]); }
+#endif
 };
 
 struct LaneStore {
@@ -41,6 +54,7 @@ struct LaneView {
   PackedFrame* stacks;
   u32* gen;
   int tt_log2;
+  long long searches;
 };
 
 __device__ __forceinline__ void lane_attach(const LaneView& v, long long lane, Lane& L, GpuStack& st) {
@@ -49,7 +63,12 @@ __device__ __forceinline__ void lane_attach(const LaneView& v, long long lane, L
   tt.tags = v.tags + (size_t)lane * buckets * kWays;
   tt.ents = v.ents + (size_t)lane * buckets * kWays;
   tt.mask = (u32)(buckets - 1);
+#if DDX_FRAMES_DEPTH_MAJOR
+  st.base = v.stacks + lane;
+  st.stride = v.searches;
+#else
   st.base = v.stacks + (size_t)lane * kStackDepth;
+#endif
   lane_init(L, tt);
   L.c.gen = v.gen[lane];
 }
@@ -63,7 +82,7 @@ int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s);
 int lanes_alloc(LaneStore& s, long long lanes, int tt_log2);
 void lanes_free(LaneStore& s);
 inline LaneView lanes_view(const LaneStore& s) {
-  return LaneView{s.tags, s.ents, s.stacks, s.gen, s.tt_log2};
+  return LaneView{s.tags, s.ents, s.stacks, s.gen, s.tt_log2, s.lanes};
 }
 
 }  // namespace ddx

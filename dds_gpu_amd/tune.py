@@ -9,13 +9,12 @@ Workloads:
          of 5, 7, 9 and 11 cards per hand and full deals in equal proportions, seeded, shuffled
 Each candidate launches one persistent solve over the whole batch (atomic job queue, captured
 HIP graph launch), runs warmup-s seconds untimed, then measure-s seconds timed from the live
-counters. Two scores, as tables per second (a job counts 1/jobs_per_deal of a table):
-  window  jobs finished in the timed window -- the bench's measure; for full deals, whose
-          batch no candidate comes near finishing
-  launch  jobs finished from the launch to the end of the window, over that time; for the
-          mixed batch, whose cheap endings the faster candidates drain during the warmup, so
-          that their timed window holds only the heaviest jobs
-The default score is window for full deals and launch for the mixed batch (--score). Every answer a candidate finishes is compared with the
+counters, then stops. Three scores, as tables per second:
+  whole   whole tables (all 20 answers written) when the stop is requested, over the seconds
+          from the launch -- the bench's value; the default
+  window  jobs finished in the timed window, a job counting 1/jobs_per_deal of a table; it
+          decays as heavy jobs pile up on the lanes, so it favours draining easy jobs
+  launch  jobs finished from the launch to the end of the window, over that time Every answer a candidate finishes is compared with the
 first candidate's answers for the same deals (all candidates are exact solvers; a difference
 is a bug and aborts the sweep).
 
@@ -93,13 +92,15 @@ def measure(cfg, hands, warmup_s, measure_s):
     j1, n1 = g.progress_nodes()
     t2 = time.perf_counter()
     g.request_stop()
+    t_stop = time.perf_counter()
     g.wait(allow_stop=True)
     g.close()
     out = d_out.cpu().numpy()
     jpd = cfg["jobs_per_deal"]
     window = (j1 - j0) / jpd / (t2 - t1)
     launch = j1 / jpd / (t2 - t0)
-    return window, out, {"tables_per_s_from_launch": round(launch, 2),
+    whole = int((out != 255).all(axis=1).sum()) / (t_stop - t0)
+    return window, out, {"whole_tables_per_s": round(whole, 2), "tables_per_s_from_launch": round(launch, 2),
                          "nodes_per_s": round((n1 - n0) / (t2 - t1)), "warm_jobs": int(j0),
                          "timed_jobs": int(j1 - j0), "startup_s": round(t1 - t0 - warmup_s, 3)}
 
@@ -111,7 +112,7 @@ def main():
     ap.add_argument("--warmup-s", type=float, default=10.0)
     ap.add_argument("--measure-s", type=float, default=10.0)
     ap.add_argument("--quick", action="store_true", help="split schedule only")
-    ap.add_argument("--score", default="", choices=["", "window", "launch"])
+    ap.add_argument("--score", default="whole", choices=["whole", "window", "launch"])
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     a = ap.parse_args()
     deals = a.deals or (100000 if a.workload == "full" else 40000)
@@ -130,8 +131,8 @@ def main():
         r = dict(cfg, tables_per_s=round(score, 2), **det)
         results.append(r)
         log(json.dumps(r))
-    score = a.score or ("window" if a.workload == "full" else "launch")
-    key = "tables_per_s" if score == "window" else "tables_per_s_from_launch"
+    score = a.score
+    key = {"whole": "whole_tables_per_s", "window": "tables_per_s", "launch": "tables_per_s_from_launch"}[score]
     best = max(results, key=lambda r: r[key])
     doc = json.load(open(a.out)) if os.path.exists(a.out) else {}
     doc[a.workload] = {"best": best, "score": score, "all": results, "deals": len(hands),
