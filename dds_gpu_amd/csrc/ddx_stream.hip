@@ -50,16 +50,37 @@ struct StreamArgs {
   int jobs_per_deal, seed_mode;
 };
 
-__device__ __forceinline__ long long load_acquire_sys(volatile long long* p) {
-  return __hip_atomic_load((long long*)p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+// Searches of a wave run their divergent branches one at a time, so a search alone in its wave
+// advances several times faster than one of four. When the ring holds little work, jobs are
+// spread over waves: the first search slot of every wave keeps a job reserved in advance, the
+// other slots reserve one only while produced jobs are waiting unreserved. Under full load
+// every slot is busy either way. DDX_STREAM_SPREAD=0 builds the plain first-come order.
+#ifndef DDX_STREAM_SPREAD
+#define DDX_STREAM_SPREAD 1
+#endif
+// Rounds between reads of the host's counters while some lane of the wave waits for input
+// (a power of two); DDX_STREAM_POLL builds another interval for measurement.
+#ifndef DDX_STREAM_POLL
+#define DDX_STREAM_POLL 256
+#endif
+
+__device__ __forceinline__ long long load_relaxed_sys(volatile long long* p) {
+  return __hip_atomic_load((long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int load_relaxed_sys(volatile int* p) {
+  return __hip_atomic_load((int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // TEAM = 1: one lane per search; 16: a 16-lane team per search. Values the host changes while
 // the kernel runs (produced, eof, stop) live in host memory: one lane of the wave reads them,
-// at most every 16 rounds and only while some lane waits for input, and broadcasts them to the
-// wave. A read crosses PCIe, and the whole wave -- its busy lanes too -- waits for it, so
-// per-lane polling every round would slow the searches most when the load is light. The
-// lanes of a team never disagree (they see one broadcast value).
+// at most every DDX_STREAM_POLL rounds and only while some lane waits for input, and
+// broadcasts them to the wave. The reads are relaxed; only when they show something new (more
+// deals, the end) does that lane add the system-scope acquire that makes the host's slot
+// contents visible. An acquire at system scope invalidates the caches the whole GPU shares,
+// TT lines included, so doing it on every poll of every wave slowed the searches most when the
+// load is light: at half load, polling every 16 rounds with an acquire each time gave a median
+// latency of 15.7 s, every 256 rounds 4.3 s. The lanes of a team never disagree (they see one
+// broadcast value).
 template <int TEAM>
 __device__ __forceinline__ void stream_body(const StreamArgs& a) {
   using TM = TeamOf<TEAM>;
@@ -78,13 +99,15 @@ __device__ __forceinline__ void stream_body(const StreamArgs& a) {
   for (;;) {
     ++rounds;
     if (((rounds & 1023) == 0 && !(seen_end & 2)) ||
-        ((rounds & 15) == 0 && __any(L.stage == ST_JOB && (pending < 0 || pending / jpd >= seen_produced)))) {
+        ((rounds & (DDX_STREAM_POLL - 1)) == 0 &&
+         __any(L.stage == ST_JOB && (pending < 0 || pending / jpd >= seen_produced)))) {
       long long v = 0;
       int e = 0;
       if (__lane_id() == 0) {  // end flags first: the host sets eof only after its last produced
-        e = (__hip_atomic_load((int*)&a.ctl->eof, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) ? 1 : 0) |
-            (__hip_atomic_load((int*)&a.ctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) ? 2 : 0);
-        v = load_acquire_sys(&a.ctl->produced);
+        e = (load_relaxed_sys(&a.ctl->eof) ? 1 : 0) | (load_relaxed_sys(&a.ctl->stop) ? 2 : 0);
+        if (e) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // eof before the last produced
+        v = load_relaxed_sys(&a.ctl->produced);
+        if (v != seen_produced) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // slots, work[]
       }
       seen_produced = __shfl(v, 0);
       seen_end = __shfl(e, 0);
@@ -113,8 +136,18 @@ __device__ __forceinline__ void stream_body(const StreamArgs& a) {
       }
     }
     if (L.stage == ST_JOB) {
-      if (pending < 0) pending = TM::take(&a.counters[0]);
-      if (pending / jpd < seen_produced) {
+      if (pending < 0) {
+        int may = 1;
+        if (DDX_STREAM_SPREAD && lane % (64 / TEAM) != 0) {
+          const unsigned long long taken =
+              __hip_atomic_load(&a.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          may = TM::bcast((int)((long long)taken < seen_produced * jpd), 0);
+        }
+        if (may) pending = TM::take(&a.counters[0]);
+      }
+      // A search without a reservation (a spread slot while no job waits) loads nothing: the
+      // test must not see -1 / jpd, which truncates to 0.
+      if (pending >= 0 && pending / jpd < seen_produced) {
         const long long p = pending / jpd;
         const long long slot = a.work[p % a.work_ring];
         lane_load_job<X>(L, B, slot * jpd + pending % jpd);  // to ROOT; runs next round
