@@ -20,8 +20,8 @@ is a bug and aborts the sweep).
 
 Swept: lanes (threads per CU x 256 CUs, with the kernel held to 3 or 4 waves per SIMD where
 the threads need it), lanes per search (one lane, or a 16-lane team), TT_LOG2
-per search (one lane: 9..12, i.e. 0.16-1.3 MB of TT per search and up to 172 GB in all; a team:
-13..14), jobs per deal (5 = one job per strain, 20 = one per lead) and the stage schedule
+per search (one lane: 9..12, i.e. 0.16-1.3 MB of TT per search and up to 258 GB in all of the
+288 GB; a team: 13..14), jobs per deal (5 = one job per strain, 20 = one per lead) and the stage schedule
 (priority / chain / split). The best is written to `out` under the workload's name; bench.py
 and the tools read the "full" entry.
 """
@@ -68,8 +68,8 @@ def candidates(quick):
                 continue
             if quick and (tpc == 256 or team == 16 or jpd == 20):
                 continue  # measured far behind on full deals: one wave per SIMD; teams; jobs per lead
-            if team == 1 and tpc * CUS * 320 * (1 << tt) > 180e9:
-                continue  # keep the TTs within ~180 GB of the 288 GB
+            if team == 1 and tpc * CUS * 320 * (1 << tt) > 260e9:
+                continue  # keep the TTs within ~260 GB of the 288 GB (the rest: stacks, batch, net)
             out.append({"lanes": CUS * tpc, "team": team, "tt_log2": tt, "schedule": sched,
                         "jobs_per_deal": jpd, "waves_per_simd": w})
     return out
