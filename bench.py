@@ -7,13 +7,17 @@
 Workload: the seeded stream 2026 of full 52-card deals (uniform Fisher-Yates, 13 cards per
 hand), D = 100,000 deals resident in HBM, 20 results per deal (uint8). Every leg launches
 one persistent solve over the whole batch; the lanes pull (deal, strain) or (deal, lead) jobs
-from an atomic queue. A "step" is S seconds of that running solve: W untimed warmup steps let
-the lanes reach a steady mix of easy and hard jobs, then K steps are timed. The live counters
-(finished jobs, nodes) are read at the step boundaries; after the last step the solve is
-stopped and synchronized. A 100,000-deal batch holds ~1e12 search nodes -- far more than a
-minute of any solver -- so throughput is the sustained rate of whole jobs, counted as
-1/jobs-per-deal of a table each (the reference's own measure, README.md:166-176, but after a
-warmup rather than from a cold start).
+from an atomic queue. A "step" is S seconds of that running solve, S = min(--step-seconds,
+--leg-seconds / (W + K)): W untimed warmup steps, then K timed steps; the live counters
+(finished jobs, nodes) are read at the step boundaries, then the solve is stopped and
+synchronized. A 100,000-deal batch holds ~1e12 search nodes with a heavy tail (single jobs of
+up to ~1e9 nodes), far more than a bench can run to completion, so:
+  value        whole tables -- deals whose 20 answers are all written -- when the stop is
+               requested, over the seconds from the launch to the stop (the ramp from a cold
+               start included; the tail still running at the stop not counted; each leg also
+               reports the jobs unfinished at the stop)
+  window_rate  jobs finished in the K timed steps, 1/jobs-per-deal of a table each (the
+               reference's own measure, README.md:166-176), which decays as heavy jobs pile up
 
 Legs (the same deals and queue order in each):
   plain    the plain wavefront kernel: 65,536 lanes, one lane per search, TT_LOG2 10 per lane,
@@ -258,15 +262,17 @@ def main():
     H = results[head]
     kern = H["kernel"]
     # HBM bytes per launch of this leg's kernel, measured by rocprofv3 --pmc FETCH_SIZE /
-    # WRITE_SIZE over this same command and scaled by a 1 GiB copy's counts (the committed
-    # profile names the command); used only when its configuration is this leg's.
+    # WRITE_SIZE over this command's learned leg and scaled by a 1 GiB copy's counts (the
+    # committed profile names the command); used only when its configuration is this leg's, and
+    # scaled by this launch's duration over the profiled one's (1.0 at the driver's step counts).
     traffic, extra = None, {}
     tpath = os.path.join(ROOT, "profiles", "solve_kernel_hbm_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("config") == H["config"] and tr.get("deals") == n:
-            traffic = tr.get("bytes_per_launch")
+        if tr.get("config") == H["config"] and tr.get("deals") == n and tr.get("launch_ms"):
+            traffic = round(tr["bytes_per_launch"] * kern["ms"] / tr["launch_ms"])
             extra = {k: tr[k] for k in ("l2_hit_rate", "valu_busy", "traffic_source") if k in tr}
+            extra["traffic_profiled_launch_ms"] = tr["launch_ms"]
     roofline = {"bound": "hbm", "achieved": kern["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(kern["achieved_gbs"] / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "kernel": "solve_kernel", "leg": head, **extra,
