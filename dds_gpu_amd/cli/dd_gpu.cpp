@@ -16,8 +16,10 @@
 //   WAVES=3|4   hold the kernel to that many waves per SIMD (split schedule); the tuned
 //               configuration for full deals is THREADS=196608 WAVES=3 with TT_LOG2 11
 //   LDS=3       keep each lane's last 3 TT tag lines in LDS (one lane per search)
-//   WAVE, STACK_KB  accepted for compatibility; the wavefront kernel is the only kernel and
-//               its frames live in HBM, so there is no stack limit to set
+//   WAVE=0      the per-thread kernel: no wave schedule, each lane runs its own next stage
+//               (DDX_SCHED_THREAD; the reference's WAVE=0 solve_kernel). Unset or 1: the
+//               wavefront kernel with SCHEDULE
+//   STACK_KB    accepted for compatibility: frames live in HBM, there is no stack limit to set
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -86,6 +88,7 @@ int main(int argc, char** argv) {
   cfg.schedule = DDX_SCHED_SPLIT;
   if (getenv("CHEAP_LOOP") && atoi(getenv("CHEAP_LOOP"))) cfg.schedule = DDX_SCHED_CHAIN;
   if (getenv("SCHEDULE")) cfg.schedule = atoi(getenv("SCHEDULE"));
+  if (getenv("WAVE") && !atoi(getenv("WAVE"))) cfg.schedule = DDX_SCHED_THREAD;
   std::vector<uint8_t> seeds;
   if (getenv("SEEDS")) {
     seeds.resize(n * 20);
@@ -115,7 +118,8 @@ int main(int argc, char** argv) {
   printf("%lld threads, ", threads);
   printf("%lld deals, %lld solves, TT %.0f KB/solve, %.2f GB total\n", n, jobs,
          320.0 * (1 << tt_log2) / 1024.0, 320.0 * (double)threads * (1 << tt_log2) / 1e9);
-  printf("wavefront kernel, schedule %d\n", cfg.schedule);
+  if (cfg.schedule == DDX_SCHED_THREAD) printf("per-thread kernel\n");
+  else printf("wavefront kernel, schedule %d\n", cfg.schedule);
   fflush(stdout);
   ddx_gpu* h = nullptr;
   if (ddx_gpu_open(&cfg, &h) != DDX_OK) {

@@ -204,7 +204,12 @@ __device__ __forceinline__ void solve_body(const KernelArgs& a) {
       reported_nodes = L.c.nodes;
     }
     if (SPLIT) split_round(L, B, st, a, rounds);
-    if (SCHED == DDX_SCHED_PRIORITY) {
+    if (SCHED == DDX_SCHED_THREAD) {
+      // No wave schedule: each lane runs its own next stage, whichever it is, so a round
+      // executes every stage some lane of the wave is in (the reference's per-thread kernel).
+      if (L.stage != ST_EXIT) lane_step<X>(L, B, st, take);
+      if (__all(L.stage == ST_EXIT)) break;
+    } else if (SCHED == DDX_SCHED_PRIORITY) {
       if (__all(L.stage == ST_EXIT)) break;
       int pick;
       if (__any(L.stage == ST_RET)) pick = ST_RET;
@@ -453,7 +458,8 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   else ddx_gpu_default_config(&cfg);
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 64 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.schedule < 0 ||
-      cfg.schedule > 2 || cfg.seed_mode < 0 || cfg.seed_mode > 3 || cfg.block < 64 ||
+      cfg.schedule > 3 || (cfg.schedule == DDX_SCHED_THREAD && cfg.profile) || cfg.seed_mode < 0 ||
+      cfg.seed_mode > 3 || cfg.block < 64 ||
       cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != 16) ||
       (cfg.waves_per_simd != 0 &&
        (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile)) ||
@@ -587,7 +593,8 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
     case 2: launch_solve<DDX_SCHED_CHAIN, false>(a, grid, block, h->cfg.team, s); break;
     case 3: launch_solve<DDX_SCHED_CHAIN, true>(a, grid, block, h->cfg.team, s); break;
     case 4: launch_solve<DDX_SCHED_SPLIT, false>(a, grid, block, h->cfg.team, s); break;
-    default: launch_solve<DDX_SCHED_SPLIT, true>(a, grid, block, h->cfg.team, s); break;
+    case 5: launch_solve<DDX_SCHED_SPLIT, true>(a, grid, block, h->cfg.team, s); break;
+    default: launch_solve<DDX_SCHED_THREAD, false>(a, grid, block, h->cfg.team, s); break;
   }
   HIPCHECK(hipGetLastError());
   return DDX_OK;

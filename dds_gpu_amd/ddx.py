@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libddx.so")
 
 SEED_PLAIN, SEED_STEP, SEED_CERT, SEED_CERTP = 0, 1, 2, 3
-SCHED_PRIORITY, SCHED_CHAIN, SCHED_SPLIT = 0, 1, 2
+SCHED_PRIORITY, SCHED_CHAIN, SCHED_SPLIT, SCHED_THREAD = 0, 1, 2, 3
 E_ARG, E_STOPPED = -1, -5
 
 _lib = None

@@ -22,7 +22,7 @@ Swept: lanes (threads per CU x 256 CUs, with the kernel held to 3 or 4 waves per
 the threads need it), lanes per search (one lane, or a 16-lane team), TT_LOG2
 per search (one lane: 9..12, i.e. 0.16-1.3 MB of TT per search and up to 258 GB in all of the
 288 GB; a team: 13..14), jobs per deal (5 = one job per strain, 20 = one per lead) and the stage schedule
-(priority / chain / split). The best is written to `out` under the workload's name; bench.py
+(priority / chain / split, or none: the per-thread kernel). The best is written to `out` under the workload's name; bench.py
 and the tools read the "full" entry.
 """
 import argparse
@@ -63,7 +63,7 @@ def candidates(quick):
     out = []
     for team, tpc, w in [(1, 256, 0), (1, 512, 0), (1, 768, 3), (1, 1024, 4), (16, 512, 0), (16, 768, 3)]:
         tts = (9, 10, 11, 12) if team == 1 else (13, 14)
-        for tt, sched, jpd in itertools.product(tts, (0, 1, 2), (5, 20)):
+        for tt, sched, jpd in itertools.product(tts, (0, 1, 2, 3), (5, 20)):
             if (quick or w) and sched != 2:
                 continue
             if quick and (tpc == 256 or team == 16 or jpd == 20):

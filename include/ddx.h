@@ -141,8 +141,11 @@ typedef struct ddx_gpu_config {
 enum {
   DDX_SCHED_PRIORITY = 0, /* one stage per round: RET > NEXT > ROOT > JOB > ENTER (reference) */
   DDX_SCHED_CHAIN = 1,    /* cheap stages run back to back per lane, then ENTER for all */
-  DDX_SCHED_SPLIT = 2     /* as CHAIN, with follow nodes entered inside the chain, so that
+  DDX_SCHED_SPLIT = 2,    /* as CHAIN, with follow nodes entered inside the chain, so that
                              trick-start nodes are entered with the whole wave gathered */
+  DDX_SCHED_THREAD = 3    /* no wave schedule: every lane runs its own next stage each round, as
+                             the reference's per-thread solve_kernel (gpu/gpu_main.cu:33-50) runs
+                             dd_solve_strain in each thread (no profile) */
 };
 
 typedef struct ddx_gpu ddx_gpu;

@@ -20,10 +20,10 @@ def c1():
 
 
 @pytest.mark.parametrize("team", [1, 16])
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 def test_c1_endings_all_schedules(ddx, c1, schedule, team):
     """4,096 8-card endings (C1), TT_LOG2 10: all 81,920 answers equal the reference's, with one
-    lane per search and with 16-lane teams."""
+    lane per search and with 16-lane teams, under every wave schedule and the per-thread one."""
     log("c1 endings, schedule", schedule, "team", team)
     with ddx.GpuSolver(lanes=8192 * team, tt_log2=10, schedule=schedule, team=team) as g:
         out, st = g.solve(c1["hands"])

@@ -16,7 +16,7 @@ def test_candidate_sets():
         per_search = 320 * (1 << c["tt_log2"])
         assert c["lanes"] // c["team"] * per_search <= 260e9 or c["team"] == 16
         assert c["waves_per_simd"] in (0, 3, 4) and (c["waves_per_simd"] == 0 or c["schedule"] == 2)
-    assert {c["schedule"] for c in full} == {0, 1, 2}  # stage orders
+    assert {c["schedule"] for c in full} == {0, 1, 2, 3}  # stage orders, and none (per-thread)
     assert {c["tt_log2"] for c in full if c["team"] == 1} == {9, 10, 11, 12}
     assert {c["lanes"] // 256 for c in full} == {256, 512, 768, 1024}  # threads per CU
     assert {c["jobs_per_deal"] for c in full} == {5, 20}
