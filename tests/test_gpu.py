@@ -237,3 +237,23 @@ def test_graph_launch_replays_match_eager(ddx, c1, oracle):
             assert st.jobs_done == 2500
             e, _ = eager.solve(hs[(i + 1) % 3])
             assert np.array_equal(e, c1["tables"][((i + 1) % 3) * 500:((i + 1) % 3 + 1) * 500])
+
+
+def test_stream_close_after_finish_stops_heavy_searches(ddx):
+    """close() after finish() stops searches still running: the four heaviest deals of the bench
+    stream (jobs of 1e8-5e8 nodes, hours on one search) are pushed, finish() marks the end of
+    input, and close() one second later returns within seconds -- the kernel keeps reading the
+    stop bit after the end of input -- with none of those deals reported."""
+    g = golden("full_2000.npz")
+    heavy = np.argsort(g["nodes"].astype(np.int64).max(axis=1))[-4:]
+    with ddx.Stream(lanes=4096, tt_log2=10, jobs_per_deal=5, ring_slots=64) as s:
+        for k, d in enumerate(heavy):
+            assert s.push(g["hands"][d], k)
+        s.finish()
+        time.sleep(1.0)
+        ids, _ = s.poll()
+        t0 = time.time()
+        s.close()
+        took = time.time() - t0
+    log("close after finish took %.2f s" % took)
+    assert took < 10 and len(ids) == 0
