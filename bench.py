@@ -264,7 +264,7 @@ def main():
     # HBM bytes per launch of this leg's kernel, measured by rocprofv3 --pmc FETCH_SIZE /
     # WRITE_SIZE over this command's learned leg and scaled by a 1 GiB copy's counts (the
     # committed profile names the command); used only when its configuration is this leg's, and
-    # scaled by this launch's duration over the profiled one's (1.0 at the driver's step counts).
+    # scaled by this launch's duration over the profiled one's (1.0 at --steps 20 --warmup 5).
     traffic, extra = None, {}
     tpath = os.path.join(ROOT, "profiles", "solve_kernel_hbm_traffic.json")
     if os.path.exists(tpath):
