@@ -65,6 +65,16 @@ struct KernelArgs {
 // one wave share its issue slots, and up to 64 would share them sixty-four ways.
 constexpr int kHelperStride = 16;
 
+// Splits start once at most lanes * DDX_SPLIT_OPEN_NUM / DDX_SPLIT_OPEN_DEN jobs of the batch are
+// unfinished: by default a quarter of the helpers (four per wave); other values build another
+// gate for measurement.
+#ifndef DDX_SPLIT_OPEN_NUM
+#define DDX_SPLIT_OPEN_NUM 1
+#endif
+#ifndef DDX_SPLIT_OPEN_DEN
+#define DDX_SPLIT_OPEN_DEN 64
+#endif
+
 // Publishes the wave's finished jobs and the nodes its searches spent up to them to its two
 // slots of host memory (plain stores by one lane), so the host reads progress without any GPU
 // work -- a copy would have to wait for a free CU while the persistent kernel holds them all.
@@ -632,7 +642,7 @@ extern "C" int ddx_gpu_solve_async(ddx_gpu* h, const uint64_t* d_hands, int64_t 
     a.sp.ctr = h->sp_ctr;
     a.sp.task_cap = a.sp.join_cap = h->sp_cap;
     a.sp.min_nodes16 = (u32)((h->cfg.split_min_nodes + 15) / 16);
-    a.sp.max_open_jobs = h->cfg.lanes / 64;  // a quarter of the helpers (four per wave)
+    a.sp.max_open_jobs = h->cfg.lanes * DDX_SPLIT_OPEN_NUM / DDX_SPLIT_OPEN_DEN;
   }
   h->t0 = std::chrono::steady_clock::now();
   HIPCHECK(hipEventRecord(h->ev0, s));

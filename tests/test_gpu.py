@@ -257,3 +257,21 @@ def test_stream_close_after_finish_stops_heavy_searches(ddx):
         took = time.time() - t0
     log("close after finish took %.2f s" % took)
     assert took < 10 and len(ids) == 0
+
+
+@pytest.mark.parametrize("schedule", [0, 2])
+def test_stage_profile_counts_are_consistent(ddx, c1, schedule):
+    """The per-stage profile (config.profile; the reference's -DDD_PROFILE output,
+    gpu/gpu_main.cu:282-301): every node entry is counted at exactly one exit point, so the exit
+    counts sum to the solve's nodes; a round's active lanes never exceed the wave's 64; cycles
+    are counted for every kind that ran; and the profiling build's tables stay exact."""
+    with ddx.GpuSolver(lanes=8192, tt_log2=10, schedule=schedule, profile=True) as g:
+        g.profile(reset=True)
+        out, st = g.solve(c1["hands"])
+        v = g.profile(reset=True).astype(np.int64)
+    assert np.array_equal(out, c1["tables"])
+    rounds, lanes, exits, cycles = v[0:8], v[8:16], v[16:23], v[24:32]
+    log("schedule %d: rounds %s, lanes %s, exits %s" % (schedule, rounds.tolist(), lanes.tolist(), exits.tolist()))
+    assert exits.sum() == st.nodes
+    assert (lanes <= 64 * rounds).all() and (lanes[rounds > 0] > 0).all()
+    assert (cycles[rounds > 0] > 0).all() and rounds[2] > 0  # ENTER (trick start) always runs
