@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--warmup-s", type=float, default=10.0)
     ap.add_argument("--measure-s", type=float, default=10.0)
     ap.add_argument("--quick", action="store_true", help="split schedule only")
+    ap.add_argument("--candidates", default="",
+                    help="a JSON list of configurations to measure instead of the built-in sets")
     ap.add_argument("--score", default="whole", choices=["whole", "window", "launch"])
     ap.add_argument("--out", default=os.path.join(HERE, "tuned.json"))
     a = ap.parse_args()
@@ -120,7 +122,7 @@ def main():
     log("workload %s: %d deals" % (a.workload, len(hands)))
     ref = None
     results = []
-    for cfg in candidates(a.quick):
+    for cfg in (json.loads(a.candidates) if a.candidates else candidates(a.quick)):
         score, out, det = measure(cfg, hands, a.warmup_s, a.measure_s)
         if ref is None:
             ref = out
