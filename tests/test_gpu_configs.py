@@ -3,8 +3,9 @@
   C2  the plain wavefront kernel (65,536 lanes, TT_LOG2 10, one job per strain) on the first
       2,000 deals of the bench stream: every answer it finishes within the budget equals the
       reference solver's table; the finished fraction is printed and must stay high. The
-      heaviest jobs of these deals cost up to 5.3e8 nodes, over an hour on one lane, so the
-      whole subset does not finish inside a test.
+      heaviest jobs of these deals cost up to 5.3e8 nodes, over an hour on one lane of this
+      configuration; tests/test_zy_c2_to_completion.py solves all 40,000 answers with 16-lane
+      teams and larger TTs.
   C3  an uneven batch (endings of 5, 7, 9 and 11 cards and full deals, shuffled) launched as a
       captured HIP graph on the tuned 3-waves-per-SIMD kernel, replayed: every answer equals
       the reference solver's, on every replay.
