@@ -5,6 +5,9 @@
 Outputs (git-ignored, shipped with the tree):
     dds_gpu_amd/lib/libddx.so            the C ABI of include/ddx.h
     dds_gpu_amd/bin/dd_cpu, dd_gpu, dd_stream, dd_train, dd_pbn, pbn, dd_verify   command lines
+    dds_gpu_amd/lib/libddx_sig.so        the same ABI built with 12-byte signature TT entries
+    dds_gpu_amd/bin/sig/dd_cpu, dd_gpu, dd_stream   its command lines
+                                         (reference: make ... NVFLAGS=-DDD_TT_SIG)
 """
 import os
 import shutil
@@ -32,6 +35,8 @@ HOST_ARCH = ["-march=x86-64-v3"]
 HIP_SOURCES = ["ddx_gpu.hip", "ddx_mlp.hip", "ddx_stream.hip"]
 HOST_SOURCES = ["ddx_host.cpp"]
 TOOLS = ["dd_cpu", "dd_gpu", "dd_stream", "dd_train", "dd_pbn", "pbn", "dd_verify"]
+# Build variants: name -> (defines, tools). "" is the default exact-entry library.
+VARIANTS = {"": ([], TOOLS), "sig": (["-DDDX_TT_SIG=1"], ["dd_cpu", "dd_gpu", "dd_stream"])}
 
 
 def _run(cmd, verbose):
@@ -43,44 +48,57 @@ def _run(cmd, verbose):
     return r.stdout
 
 
-def _objects(verbose):
-    os.makedirs(OBJ, exist_ok=True)
+def _compile_jobs(variant):
+    defines = VARIANTS[variant][0]
+    obj = os.path.join(OBJ, variant) if variant else OBJ
+    os.makedirs(obj, exist_ok=True)
     jobs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        out = os.path.join(OBJ, src + ".o")
-        jobs.append([HIPCC] + DEVICE_FLAGS + COMMON + ["-c", path, "-o", out])
+        out = os.path.join(obj, src + ".o")
+        jobs.append([HIPCC] + DEVICE_FLAGS + COMMON + defines + ["-c", path, "-o", out])
     for src in HOST_SOURCES:
-        out = os.path.join(OBJ, src + ".o")
-        jobs.append([CXX] + COMMON + HOST_ARCH + ["-c", os.path.join(CSRC, src), "-o", out])
-    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        list(ex.map(lambda c: _run(c, verbose), jobs))
-    return [j[-1] for j in jobs]
+        out = os.path.join(obj, src + ".o")
+        jobs.append([CXX] + COMMON + HOST_ARCH + defines + ["-c", os.path.join(CSRC, src), "-o", out])
+    return jobs
 
 
-def build(verbose=False):
-    """Compiles every HIP source for gfx950 and links libddx.so and the tools."""
-    objs = _objects(verbose)
-    os.makedirs(LIB, exist_ok=True)
-    os.makedirs(BIN, exist_ok=True)
-    lib = os.path.join(LIB, "libddx.so")
+def _link(variant, objs, verbose):
+    suffix = "_" + variant if variant else ""
+    lib = os.path.join(LIB, "libddx%s.so" % suffix)
     tmp = lib + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, lib)
+    bindir = os.path.join(BIN, variant) if variant else BIN
+    os.makedirs(bindir, exist_ok=True)
+    rpath = "$ORIGIN/../../lib" if variant else "$ORIGIN/../lib"
     tools = []
-    for t in TOOLS:
+    for t in VARIANTS[variant][1]:
         src = os.path.join(PKG, "cli", t + ".cpp")
         if not os.path.exists(src):
             continue
-        out = os.path.join(BIN, t)
-        tools.append([CXX] + COMMON + HOST_ARCH + [src, "-o", out, "-L" + LIB, "-lddx",
-                                                     "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"])
-    if tools:
-        with ThreadPoolExecutor(max_workers=min(8, len(tools))) as ex:
-            list(ex.map(lambda c: _run(c, verbose), tools))
-    return lib
+        tools.append([CXX] + COMMON + HOST_ARCH + VARIANTS[variant][0] +
+                     [src, "-o", os.path.join(bindir, t), "-L" + LIB, "-lddx" + suffix,
+                      "-Wl,-rpath," + rpath, "-lpthread"])
+    return lib, tools
+
+
+def build(verbose=False):
+    """Compiles every HIP source for gfx950 and links libddx.so, its signature-entry variant
+    libddx_sig.so, and the tools."""
+    os.makedirs(LIB, exist_ok=True)
+    jobs = {v: _compile_jobs(v) for v in VARIANTS}
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        list(ex.map(lambda c: _run(c, verbose), [c for v in VARIANTS for c in jobs[v]]))
+    tools = []
+    for v in VARIANTS:
+        lib, t = _link(v, [c[-1] for c in jobs[v]], verbose)
+        tools += t
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        list(ex.map(lambda c: _run(c, verbose), tools))
+    return os.path.join(LIB, "libddx.so")
 
 
 if __name__ == "__main__":

@@ -348,7 +348,7 @@ int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s) {
 
 int lanes_alloc(LaneStore& s, long long lanes, int tt_log2) {
   const size_t L = (size_t)lanes, buckets = (size_t)1 << tt_log2;
-  const size_t tag_bytes = L * buckets * 64, ent_bytes = L * buckets * 256;
+  const size_t tag_bytes = L * buckets * 64, ent_bytes = L * buckets * kBucketEntryBytes;
   const size_t stack_bytes = L * kStackDepth * sizeof(PackedFrame), gen_bytes = (L * 4 + 15) / 16 * 16;
   s.lanes = lanes;
   s.tt_log2 = tt_log2;
@@ -466,11 +466,15 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   ddx_gpu_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ddx_gpu_default_config(&cfg);
+  if (DDX_TT_SIG && cfg.team != 1) {
+    ddx_set_error("ddx_gpu_open: the signature-entry build runs one lane per search (team 1)");
+    return DDX_E_ARG;
+  }
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 64 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.schedule < 0 ||
       cfg.schedule > 3 || (cfg.schedule == DDX_SCHED_THREAD && cfg.profile) || cfg.seed_mode < 0 ||
       cfg.seed_mode > 3 || cfg.block < 64 ||
-      cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != 16) ||
+      cfg.block > 256 || cfg.block % 64 || cfg.lanes % cfg.block || (cfg.team != 1 && cfg.team != kWideTeam) ||
       (cfg.waves_per_simd != 0 &&
        (cfg.waves_per_simd < 3 || cfg.waves_per_simd > 4 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile)) ||
       (cfg.lds_lines != 0 && (cfg.lds_lines != 3 || cfg.team != 1 || cfg.schedule != DDX_SCHED_SPLIT ||
@@ -553,7 +557,7 @@ extern "C" double ddx_gpu_last_kernel_ms(const ddx_gpu* h) { return h ? h->last_
 
 template <int S, bool P>
 static void launch_solve(const KernelArgs& a, int grid, int block, int team, hipStream_t s) {
-  if (team == 16) hipLaunchKernelGGL((solve_kernel<S, P, 16>), dim3(grid), dim3(block), 0, s, a);
+  if (team == 16) hipLaunchKernelGGL((solve_kernel<S, P, kWideTeam>), dim3(grid), dim3(block), 0, s, a);
   else hipLaunchKernelGGL((solve_kernel<S, P, 1>), dim3(grid), dim3(block), 0, s, a);
 }
 
@@ -590,9 +594,9 @@ static int enqueue_solve(ddx_gpu* h, const KernelArgs& a, int64_t n, hipStream_t
     return DDX_OK;
   }
   if (W == 3 || W == 4) {  // split schedule, no profile (checked at open)
-    if (W == 3 && h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w3<16>, dim3(grid), dim3(block), 0, s, a);
+    if (W == 3 && h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w3<kWideTeam>, dim3(grid), dim3(block), 0, s, a);
     else if (W == 3) hipLaunchKernelGGL(solve_kernel_w3<1>, dim3(grid), dim3(block), 0, s, a);
-    else if (h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w4<16>, dim3(grid), dim3(block), 0, s, a);
+    else if (h->cfg.team == 16) hipLaunchKernelGGL(solve_kernel_w4<kWideTeam>, dim3(grid), dim3(block), 0, s, a);
     else hipLaunchKernelGGL(solve_kernel_w4<1>, dim3(grid), dim3(block), 0, s, a);
     HIPCHECK(hipGetLastError());
     return DDX_OK;

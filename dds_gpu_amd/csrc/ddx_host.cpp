@@ -33,7 +33,9 @@ void ddx_set_error(const char* fmt, ...) {
 }
 
 extern "C" const char* ddx_last_error(void) { return g_err.c_str(); }
-extern "C" const char* ddx_version(void) { return "ddx 0.1 (gfx950)"; }
+extern "C" const char* ddx_version(void) {
+  return DDX_TT_SIG ? "ddx 0.1 (gfx950, 12-byte signature TT entries)" : "ddx 0.1 (gfx950, exact TT entries)";
+}
 
 extern "C" int ddx_random_deals(uint64_t seed, int64_t first, int64_t n, int k, uint64_t* hands) {
   if (n < 0 || first < 0 || k < 1 || k > 13 || (!hands && n)) {

@@ -61,7 +61,7 @@ __device__ __forceinline__ void lane_attach(const LaneView& v, long long lane, L
   const size_t buckets = (size_t)1 << v.tt_log2;
   TTView tt;
   tt.tags = v.tags + (size_t)lane * buckets * kWays;
-  tt.ents = v.ents + (size_t)lane * buckets * kWays;
+  tt.ents = (Entry*)((char*)v.ents + (size_t)lane * buckets * kBucketEntryBytes);
   tt.mask = (u32)(buckets - 1);
 #if DDX_FRAMES_DEPTH_MAJOR
   st.base = v.stacks + lane;

@@ -248,9 +248,13 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   ddx_gpu_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ddx_gpu_default_config(&cfg);
+  if (DDX_TT_SIG && cfg.team != 1) {
+    ddx_set_error("ddx_stream_open: the signature-entry build runs one lane per search (team 1)");
+    return DDX_E_ARG;
+  }
   if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 256 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3 ||
-      (cfg.team != 1 && cfg.team != 16) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3) ||
+      (cfg.team != 1 && cfg.team != kWideTeam) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3) ||
       cfg.split_min_nodes != 0) {
     ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, 20], "
                   "jobs_per_deal 5 or 20; the stream kernel does not split searches)");
@@ -316,10 +320,10 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   s->t0 = std::chrono::steady_clock::now();
   const dim3 grid((unsigned)(cfg.lanes / 256));
   if (cfg.waves_per_simd == 3) {
-    if (cfg.team == 16) hipLaunchKernelGGL(stream_kernel_w3<16>, grid, dim3(256), 0, s->stream, a);
+    if (cfg.team == 16) hipLaunchKernelGGL(stream_kernel_w3<kWideTeam>, grid, dim3(256), 0, s->stream, a);
     else hipLaunchKernelGGL(stream_kernel_w3<1>, grid, dim3(256), 0, s->stream, a);
   } else if (cfg.team == 16) {
-    hipLaunchKernelGGL(stream_kernel<16>, grid, dim3(256), 0, s->stream, a);
+    hipLaunchKernelGGL(stream_kernel<kWideTeam>, grid, dim3(256), 0, s->stream, a);
   } else {
     hipLaunchKernelGGL(stream_kernel<1>, grid, dim3(256), 0, s->stream, a);
   }

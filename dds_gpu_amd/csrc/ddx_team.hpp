@@ -202,6 +202,10 @@ struct Team16 {
 // the team-wide job counter / broadcast used by the kernels.
 template <int TEAM>
 struct TeamOf;
+
+// Team size of the wide kernels: 16, or 1 in the signature-entry build (Team16's TT code
+// reads and writes exact entries), whose open calls then reject team = 16.
+constexpr int kWideTeam = DDX_TT_SIG ? 1 : 16;
 template <>
 struct TeamOf<1> {
   using Ops = SeqOps;

@@ -10,12 +10,15 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libddx.so")
+# Build variants of the same C ABI: "" the default exact TT entries, "sig" 12-byte signature
+# entries (the reference's -DDD_TT_SIG build).
+LIB_PATHS = {"": LIB_PATH, "sig": os.path.join(_HERE, "lib", "libddx_sig.so")}
 
 SEED_PLAIN, SEED_STEP, SEED_CERT, SEED_CERTP = 0, 1, 2, 3
 SCHED_PRIORITY, SCHED_CHAIN, SCHED_SPLIT, SCHED_THREAD = 0, 1, 2, 3
 E_ARG, E_STOPPED = -1, -5
 
-_lib = None
+_libs = {}
 
 
 class Stats(ctypes.Structure):
@@ -43,14 +46,15 @@ class DdxError(RuntimeError):
         self.rc = rc
 
 
-def lib():
-    """Loads libddx.so (built by dds_gpu_amd.build); raises if it is missing."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libddx.so is not built: run `python -m dds_gpu_amd.build` "
-                               "(expected %s)" % LIB_PATH)
-        L = ctypes.CDLL(LIB_PATH)
+def lib(variant=""):
+    """Loads libddx.so, or a build variant of it (built by dds_gpu_amd.build); raises if it is
+    missing."""
+    if variant not in _libs:
+        path = LIB_PATHS[variant]
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built: run `python -m dds_gpu_amd.build` "
+                               "(expected %s)" % (os.path.basename(path), path))
+        L = ctypes.CDLL(path)
         P, I64, U64, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int
         L.ddx_last_error.restype = ctypes.c_char_p
         L.ddx_version.restype = ctypes.c_char_p
@@ -97,13 +101,13 @@ def lib():
         L.ddx_mlp_predict_noalloc.argtypes = [P, P, I64, P, P]
         L.ddx_mlp_save.argtypes = [P, ctypes.c_char_p]
         L.ddx_mlp_load.argtypes = [P, ctypes.c_char_p]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
-def _check(rc, ok=(0,)):
+def _check(rc, ok=(0,), L=None):
     if rc not in ok:
-        raise DdxError(rc, lib().ddx_last_error().decode())
+        raise DdxError(rc, (L or lib()).ddx_last_error().decode())
     return rc
 
 
@@ -137,7 +141,7 @@ def pgx_write(path, hands, tables=None):
 
 
 def cpu_solve(hands, threads=1, tt_log2=10, jobs_per_deal=5, seed_mode=SEED_PLAIN, seeds=None,
-              job_nodes=False):
+              job_nodes=False, variant=""):
     """Tables (n, 20) of the CPU solver; with job_nodes (threads == 1) also the nodes per job."""
     hands = np.ascontiguousarray(hands, np.uint64)
     n = len(hands)
@@ -145,41 +149,46 @@ def cpu_solve(hands, threads=1, tt_log2=10, jobs_per_deal=5, seed_mode=SEED_PLAI
     nodes = np.zeros(n * jobs_per_deal, np.uint64) if job_nodes else None
     s = None if seeds is None else np.ascontiguousarray(seeds, np.uint8)
     st = Stats()
-    _check(lib().ddx_cpu_solve_batch(_ptr(hands), n, threads, tt_log2, jobs_per_deal, seed_mode,
-                                     _ptr(s), _ptr(out), _ptr(nodes), ctypes.byref(st)))
+    L = lib(variant)
+    _check(L.ddx_cpu_solve_batch(_ptr(hands), n, threads, tt_log2, jobs_per_deal, seed_mode,
+                                 _ptr(s), _ptr(out), _ptr(nodes), ctypes.byref(st)), L=L)
     return (out, st, nodes) if job_nodes else (out, st)
 
 
 class CpuContext:
     """One CPU solver context (reference DDCtx + its TT)."""
 
-    def __init__(self, tt_log2=12):
+    def _ck(self, rc, ok=(0,)):
+        return _check(rc, ok, self._L)
+
+    def __init__(self, tt_log2=12, variant=""):
+        self._L = lib(variant)
         self.h = ctypes.c_void_p()
-        _check(lib().ddx_cpu_create(tt_log2, ctypes.byref(self.h)))
+        self._ck(self._L.ddx_cpu_create(tt_log2, ctypes.byref(self.h)))
 
     def solve_strain(self, hands, strain, ns4=None):
         hands = np.ascontiguousarray(hands, np.uint64)
         out = np.zeros(4, np.int32)
         if ns4 is None:
-            _check(lib().ddx_cpu_solve_strain(self.h, _ptr(hands), strain, _ptr(out)))
+            self._ck(self._L.ddx_cpu_solve_strain(self.h, _ptr(hands), strain, _ptr(out)))
         else:
             g = np.ascontiguousarray(ns4, np.int32)
-            _check(lib().ddx_cpu_solve_strain_guess(self.h, _ptr(hands), strain, _ptr(g), _ptr(out)))
+            self._ck(self._L.ddx_cpu_solve_strain_guess(self.h, _ptr(hands), strain, _ptr(g), _ptr(out)))
         return out
 
     def ns_tricks(self, hands, strain, leader, guess=7):
         hands = np.ascontiguousarray(hands, np.uint64)
         v = ctypes.c_int()
-        _check(lib().ddx_cpu_ns_tricks(self.h, _ptr(hands), strain, leader, guess, ctypes.byref(v)))
+        self._ck(self._L.ddx_cpu_ns_tricks(self.h, _ptr(hands), strain, leader, guess, ctypes.byref(v)))
         return v.value
 
     @property
     def nodes(self):
-        return lib().ddx_cpu_nodes(self.h)
+        return self._L.ddx_cpu_nodes(self.h)
 
     def close(self):
         if self.h:
-            lib().ddx_cpu_destroy(self.h)
+            self._L.ddx_cpu_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -192,11 +201,15 @@ class CpuContext:
 class GpuSolver:
     """A set of solver lanes on one GPU (ddx_gpu_open / ddx_gpu_solve)."""
 
+    def _ck(self, rc, ok=(0,)):
+        return _check(rc, ok, self._L)
+
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=5, schedule=SCHED_CHAIN,
                  seed_mode=SEED_PLAIN, block=256, profile=False, device=0, team=1, graph=False,
-                 waves_per_simd=0, lds_lines=0, split_min_nodes=0):
+                 waves_per_simd=0, lds_lines=0, split_min_nodes=0, variant=""):
+        self._L = lib(variant)
         cfg = GpuConfig()
-        lib().ddx_gpu_default_config(ctypes.byref(cfg))
+        self._L.ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2 = device, lanes, tt_log2
         cfg.jobs_per_deal, cfg.schedule, cfg.seed_mode = jobs_per_deal, schedule, seed_mode
         cfg.block, cfg.profile, cfg.team, cfg.graph = block, int(profile), team, int(graph)
@@ -205,7 +218,7 @@ class GpuSolver:
         cfg.split_min_nodes = split_min_nodes
         self.cfg = cfg
         self.h = ctypes.c_void_p()
-        _check(lib().ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))
+        self._ck(self._L.ddx_gpu_open(ctypes.byref(cfg), ctypes.byref(self.h)))
 
     def solve(self, hands, seeds=None, allow_stop=False):
         hands = np.ascontiguousarray(hands, np.uint64)
@@ -213,51 +226,51 @@ class GpuSolver:
         out = np.zeros((n, 20), np.uint8)
         s = None if seeds is None else np.ascontiguousarray(seeds, np.uint8)
         st = Stats()
-        _check(lib().ddx_gpu_solve(self.h, _ptr(hands), n, _ptr(s), _ptr(out), ctypes.byref(st)),
+        self._ck(self._L.ddx_gpu_solve(self.h, _ptr(hands), n, _ptr(s), _ptr(out), ctypes.byref(st)),
                ok=(0, E_STOPPED) if allow_stop else (0,))
         return out, st
 
     def solve_async(self, d_hands, n, d_out, d_seeds=0, stream=0):
         """Device-resident batch (raw device pointers as ints); then wait()."""
-        _check(lib().ddx_gpu_solve_async(self.h, ctypes.c_void_p(d_hands), n,
+        self._ck(self._L.ddx_gpu_solve_async(self.h, ctypes.c_void_p(d_hands), n,
                                          ctypes.c_void_p(d_seeds) if d_seeds else None,
                                          ctypes.c_void_p(d_out), ctypes.c_void_p(stream) if stream else None))
 
     def wait(self, allow_stop=False):
         st = Stats()
-        _check(lib().ddx_gpu_wait(self.h, ctypes.byref(st)), ok=(0, E_STOPPED) if allow_stop else (0,))
+        self._ck(self._L.ddx_gpu_wait(self.h, ctypes.byref(st)), ok=(0, E_STOPPED) if allow_stop else (0,))
         return st
 
     def request_stop(self):
-        lib().ddx_gpu_request_stop(self.h)
+        self._L.ddx_gpu_request_stop(self.h)
 
     @property
     def last_kernel_ms(self):
-        return lib().ddx_gpu_last_kernel_ms(self.h)
+        return self._L.ddx_gpu_last_kernel_ms(self.h)
 
     @property
     def device_bytes(self):
-        return lib().ddx_gpu_device_bytes(self.h)
+        return self._L.ddx_gpu_device_bytes(self.h)
 
     def progress(self):
         v = ctypes.c_uint64()
-        _check(lib().ddx_gpu_progress(self.h, ctypes.byref(v)))
+        self._ck(self._L.ddx_gpu_progress(self.h, ctypes.byref(v)))
         return v.value
 
     def progress_nodes(self):
         """(jobs finished, nodes spent up to each search's last finished job) of a running solve."""
         v, w = ctypes.c_uint64(), ctypes.c_uint64()
-        _check(lib().ddx_gpu_progress_nodes(self.h, ctypes.byref(v), ctypes.byref(w)))
+        self._ck(self._L.ddx_gpu_progress_nodes(self.h, ctypes.byref(v), ctypes.byref(w)))
         return v.value, w.value
 
     def profile(self, reset=True):
         v = np.zeros(32, np.uint64)
-        _check(lib().ddx_gpu_profile(self.h, _ptr(v), int(reset)))
+        self._ck(self._L.ddx_gpu_profile(self.h, _ptr(v), int(reset)))
         return v
 
     def close(self):
         if self.h:
-            lib().ddx_gpu_close(self.h)
+            self._L.ddx_gpu_close(self.h)
             self.h = ctypes.c_void_p()
 
     def __enter__(self):
@@ -287,15 +300,19 @@ E_FULL = 1
 class Stream:
     """The persistent streaming solver (ddx_stream_*): push deals, poll finished tables."""
 
+    def _ck(self, rc, ok=(0,)):
+        return _check(rc, ok, self._L)
+
     def __init__(self, lanes=65536, tt_log2=10, jobs_per_deal=20, seed_mode=SEED_PLAIN,
-                 ring_slots=4096, device=0, team=1, waves_per_simd=0):
+                 ring_slots=4096, device=0, team=1, waves_per_simd=0, variant=""):
+        self._L = lib(variant)
         cfg = GpuConfig()
-        lib().ddx_gpu_default_config(ctypes.byref(cfg))
+        self._L.ddx_gpu_default_config(ctypes.byref(cfg))
         cfg.device, cfg.lanes, cfg.tt_log2, cfg.team = device, lanes, tt_log2, team
         cfg.waves_per_simd = waves_per_simd
         cfg.jobs_per_deal, cfg.seed_mode = jobs_per_deal, seed_mode
         self.h = ctypes.c_void_p()
-        _check(lib().ddx_stream_open(ctypes.byref(cfg), ring_slots, ctypes.byref(self.h)))
+        self._ck(self._L.ddx_stream_open(ctypes.byref(cfg), ring_slots, ctypes.byref(self.h)))
         self._ids = np.zeros(4096, np.int64)
         self._tab = np.zeros((4096, 20), np.uint8)
 
@@ -303,27 +320,27 @@ class Stream:
         """True if queued, False if the ring is full."""
         h = np.ascontiguousarray(hands4, np.uint64)
         s = None if seeds20 is None else np.ascontiguousarray(seeds20, np.uint8)
-        rc = _check(lib().ddx_stream_push(self.h, _ptr(h), _ptr(s), deal_id), ok=(0, E_FULL))
+        rc = self._ck(self._L.ddx_stream_push(self.h, _ptr(h), _ptr(s), deal_id), ok=(0, E_FULL))
         return rc == 0
 
     def poll(self):
         got = ctypes.c_int()
-        _check(lib().ddx_stream_poll(self.h, len(self._ids), _ptr(self._ids), _ptr(self._tab),
+        self._ck(self._L.ddx_stream_poll(self.h, len(self._ids), _ptr(self._ids), _ptr(self._tab),
                                      ctypes.byref(got)))
         n = got.value
         return self._ids[:n].copy(), self._tab[:n].copy()
 
     def in_flight(self):
-        return lib().ddx_stream_in_flight(self.h)
+        return self._L.ddx_stream_in_flight(self.h)
 
     def finish(self):
-        _check(lib().ddx_stream_finish(self.h))
+        self._ck(self._L.ddx_stream_finish(self.h))
 
     def close(self):
         st = Stats()
         if self.h:
             h, self.h = self.h, ctypes.c_void_p()
-            _check(lib().ddx_stream_close(h, ctypes.byref(st)))
+            self._ck(self._L.ddx_stream_close(h, ctypes.byref(st)))
         return st
 
     def __enter__(self):

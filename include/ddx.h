@@ -56,7 +56,8 @@ typedef struct ddx_stats {
 /* Message of the last failure on this thread ("" if none). */
 const char* ddx_last_error(void);
 
-/* Library version string. */
+/* Library version string; it names the TT entry format the library was built with (exact
+ * 16-byte entries, or 12-byte signature entries: the reference's -DDD_TT_SIG build). */
 const char* ddx_version(void);
 
 /* ---- deals and files ------------------------------------------------------------------ */

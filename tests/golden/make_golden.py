@@ -23,6 +23,10 @@ Fixtures:
                    Apache-2.0, Pgx authors), kept as data.
   mixed_endings.npz  300 random endings each of 5, 7, 9 and 11 cards per hand (seed 3000 + K),
                    reference tables at TT_LOG2 10: the endings of the uneven (mixed) batch.
+  sig_entries.npz  the reference built with 12-byte signature TT entries (-DDD_TT_SIG,
+                   oracle/_ref/ref_harness_sig), TT_LOG2 10, one context per group, jobs in
+                   index order: tables and job nodes of the first 400 C1 endings, the first
+                   100 11-card mixed endings and 12 full deals of full_2000.npz.
 """
 import argparse
 import os
@@ -41,10 +45,12 @@ C1_SEED, C1_K, C1_N = 1, 8, 4096
 C2_SEED, FULL_N, FULL_CHUNK = 2026, 2000, 250
 FUNCS_N, FUNCS_SEED = 6000, 77
 MIXED_KS, MIXED_N, MIXED_SEED = (5, 7, 9, 11), 300, 3000
+SIG_C1_N, SIG_K11_N = 400, 100
+SIG_FULL = (0, 1, 2, 4, 5, 6, 7, 8, 10, 12, 14, 15)  # full_2000 rows below 3e6 reference nodes
 
 
-def harness(ref, *args):
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+def harness(ref, *args, exe="ref_harness"):
+    exe = os.path.join(ROOT, "oracle", "_ref", exe)
     subprocess.check_call([exe] + [str(a) for a in args])
 
 
@@ -53,7 +59,7 @@ def deals(seed, first, n, k):
     return ddx.random_deals(seed, n, k, first)
 
 
-def ref_tables(ref, hands, tt_log2, seeds=None):
+def ref_tables(ref, hands, tt_log2, seeds=None, exe="ref_harness"):
     with tempfile.TemporaryDirectory() as d:
         hp, tp, npth = (os.path.join(d, x) for x in ("h.bin", "t.bin", "n.bin"))
         np.ascontiguousarray(hands, np.uint64).tofile(hp)
@@ -62,7 +68,7 @@ def ref_tables(ref, hands, tt_log2, seeds=None):
             sp = os.path.join(d, "s.bin")
             np.ascontiguousarray(seeds, np.uint8).tofile(sp)
             args.append(sp)
-        harness(ref, *args)
+        harness(ref, *args, exe=exe)
         t = np.fromfile(tp, np.uint8).reshape(-1, 20)
         nodes = np.fromfile(npth, np.uint64).reshape(-1, 5)
     return t, nodes
@@ -113,6 +119,19 @@ def make_mixed(ref):
                         seed=MIXED_SEED, tt_log2=10)
 
 
+def make_sig(ref):
+    groups = {"c1": deals(C1_SEED, 0, SIG_C1_N, C1_K),
+              "k11": deals(MIXED_SEED + 11, 0, SIG_K11_N, 11),
+              "full": deals(C2_SEED, 0, FULL_N, 13)[list(SIG_FULL)]}
+    with ThreadPoolExecutor(max_workers=len(groups)) as ex:
+        res = dict(zip(groups, ex.map(lambda h: ref_tables(ref, h, 10, exe="ref_harness_sig"), groups.values())))
+    out = {}
+    for name, h in groups.items():
+        out[name + "_hands"] = h
+        out[name + "_tables"], out[name + "_nodes"] = res[name]
+    np.savez_compressed(os.path.join(HERE, "sig_entries.npz"), full_rows=np.array(SIG_FULL), tt_log2=10, **out)
+
+
 def make_funcs(ref):
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "f.bin")
@@ -132,7 +151,8 @@ def main():
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "REF=" + a.ref])
-    jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full, "mixed": make_mixed}
+    jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full, "mixed": make_mixed,
+            "sig": make_sig}
     for name, fn in jobs.items():
         if a.only and name not in a.only.split(","):
             continue

@@ -4,6 +4,7 @@ import os
 import re
 import subprocess
 
+import pytest
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "ddx.h")
@@ -23,11 +24,13 @@ def test_header_declares_the_boundary():
         assert must in names, must
 
 
-def test_library_exports_every_declared_symbol(ddx):
-    lib = ddx.lib()
+@pytest.mark.parametrize("variant", ["", "sig"])
+def test_library_exports_every_declared_symbol(ddx, variant):
+    """Both builds (exact TT entries, and 12-byte signature entries) export the whole ABI."""
+    lib = ddx.lib(variant)
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
-    out = subprocess.run(["nm", "-D", "--defined-only", ddx.LIB_PATH], capture_output=True,
+    out = subprocess.run(["nm", "-D", "--defined-only", ddx.LIB_PATHS[variant]], capture_output=True,
                          text=True).stdout
     exported = set(re.findall(r"\bT (ddx_\w+)", out))
     assert set(declared()) <= exported
