@@ -8,6 +8,8 @@ Outputs (git-ignored, shipped with the tree):
     dds_gpu_amd/lib/libddx_sig.so        the same ABI built with 12-byte signature TT entries
     dds_gpu_amd/bin/sig/dd_cpu, dd_gpu, dd_stream   its command lines
                                          (reference: make ... NVFLAGS=-DDD_TT_SIG)
+    dds_gpu_amd/lib/libddx_shared.so     the same ABI with one lock-free TT shared by all searches
+    dds_gpu_amd/bin/shared/dd_cpu, dd_gpu, dd_stream   its command lines (reference: -DDD_SHARED_TT)
 """
 import os
 import shutil
@@ -36,7 +38,8 @@ HIP_SOURCES = ["ddx_gpu.hip", "ddx_mlp.hip", "ddx_stream.hip"]
 HOST_SOURCES = ["ddx_host.cpp"]
 TOOLS = ["dd_cpu", "dd_gpu", "dd_stream", "dd_train", "dd_pbn", "pbn", "dd_verify"]
 # Build variants: name -> (defines, tools). "" is the default exact-entry library.
-VARIANTS = {"": ([], TOOLS), "sig": (["-DDDX_TT_SIG=1"], ["dd_cpu", "dd_gpu", "dd_stream"])}
+VARIANTS = {"": ([], TOOLS), "sig": (["-DDDX_TT_SIG=1"], ["dd_cpu", "dd_gpu", "dd_stream"]),
+            "shared": (["-DDDX_SHARED_TT=1"], ["dd_cpu", "dd_gpu", "dd_stream"])}
 
 
 def _run(cmd, verbose):
@@ -86,8 +89,8 @@ def _link(variant, objs, verbose):
 
 
 def build(verbose=False):
-    """Compiles every HIP source for gfx950 and links libddx.so, its signature-entry variant
-    libddx_sig.so, and the tools."""
+    """Compiles every HIP source for gfx950 and links libddx.so, its signature-entry and
+    shared-TT variants (libddx_sig.so, libddx_shared.so), and the tools."""
     os.makedirs(LIB, exist_ok=True)
     jobs = {v: _compile_jobs(v) for v in VARIANTS}
     with ThreadPoolExecutor(max_workers=8) as ex:

@@ -348,7 +348,8 @@ int launch_fill(void* p, size_t bytes, u32 v, hipStream_t s) {
 
 int lanes_alloc(LaneStore& s, long long lanes, int tt_log2) {
   const size_t L = (size_t)lanes, buckets = (size_t)1 << tt_log2;
-  const size_t tag_bytes = L * buckets * 64, ent_bytes = L * buckets * kBucketEntryBytes;
+  const size_t tts = DDX_SHARED_TT ? 1 : L;  // TTs: one per search, or one shared
+  const size_t tag_bytes = tts * buckets * 64, ent_bytes = tts * buckets * kBucketEntryBytes;
   const size_t stack_bytes = L * kStackDepth * sizeof(PackedFrame), gen_bytes = (L * 4 + 15) / 16 * 16;
   s.lanes = lanes;
   s.tt_log2 = tt_log2;
@@ -466,11 +467,16 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
   ddx_gpu_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ddx_gpu_default_config(&cfg);
-  if (DDX_TT_SIG && cfg.team != 1) {
-    ddx_set_error("ddx_gpu_open: the signature-entry build runs one lane per search (team 1)");
+  if ((DDX_TT_SIG || DDX_SHARED_TT) && cfg.team != 1) {
+    ddx_set_error("ddx_gpu_open: the %s build runs one lane per search (team 1)",
+                  DDX_TT_SIG ? "signature-entry" : "shared-TT");
     return DDX_E_ARG;
   }
-  if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 64 ||
+  if (DDX_SHARED_TT && (cfg.split_min_nodes || cfg.lds_lines)) {
+    ddx_set_error("ddx_gpu_open: the shared-TT build has no splitting and no LDS line cache");
+    return DDX_E_ARG;
+  }
+  if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > kMaxTTLog2 || cfg.lanes < 64 || cfg.lanes % 64 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.schedule < 0 ||
       cfg.schedule > 3 || (cfg.schedule == DDX_SCHED_THREAD && cfg.profile) || cfg.seed_mode < 0 ||
       cfg.seed_mode > 3 || cfg.block < 64 ||
@@ -483,9 +489,9 @@ extern "C" int ddx_gpu_open(const ddx_gpu_config* cfg_in, ddx_gpu** out) {
       (cfg.split_min_nodes > 0 && (cfg.team != 1 || cfg.schedule != DDX_SCHED_SPLIT || cfg.profile ||
                                    cfg.lds_lines || cfg.waves_per_simd == 4))) {
     ddx_set_error("ddx_gpu_open: bad config (lanes %lld must be a multiple of block %d and of 64, "
-                  "TT_LOG2 %d in [9, 20], jobs_per_deal 5 or 20, team 1 or 16; splitting needs one "
+                  "TT_LOG2 %d in [9, %d], jobs_per_deal 5 or 20, team 1 or 16; splitting needs one "
                   "lane per search, the split schedule, no profile, no LDS cache, 2 or 3 waves per SIMD)",
-                  (long long)cfg.lanes, cfg.block, cfg.tt_log2);
+                  (long long)cfg.lanes, cfg.block, cfg.tt_log2, kMaxTTLog2);
     return DDX_E_ARG;
   }
   int ndev = 0;

@@ -34,7 +34,9 @@ void ddx_set_error(const char* fmt, ...) {
 
 extern "C" const char* ddx_last_error(void) { return g_err.c_str(); }
 extern "C" const char* ddx_version(void) {
-  return DDX_TT_SIG ? "ddx 0.1 (gfx950, 12-byte signature TT entries)" : "ddx 0.1 (gfx950, exact TT entries)";
+  return DDX_TT_SIG      ? "ddx 0.1 (gfx950, 12-byte signature TT entries)"
+         : DDX_SHARED_TT ? "ddx 0.1 (gfx950, one shared lock-free TT, 32-byte checksummed entries)"
+                         : "ddx 0.1 (gfx950, exact TT entries)";
 }
 
 extern "C" int ddx_random_deals(uint64_t seed, int64_t first, int64_t n, int k, uint64_t* hands) {
@@ -125,14 +127,14 @@ static bool make_tt(int tt_log2, std::vector<u32>& tags, std::vector<Entry>& ent
   size_t buckets = (size_t)1 << tt_log2;
   try {
     tags.assign(buckets * kWays + 16, 0);
-    ents.assign(buckets * kWays, Entry{0, 0});
+    ents.assign(buckets * (kBucketEntryBytes / sizeof(Entry)) + 4, Entry{0, 0});
   } catch (const std::bad_alloc&) {
     return false;
   }
-  // 64-byte align the tag lines.
-  uintptr_t p = (uintptr_t)tags.data();
+  // 64-byte align the tag lines and the entries.
+  uintptr_t p = (uintptr_t)tags.data(), q = (uintptr_t)ents.data();
   v.tags = (u32*)((p + 63) & ~(uintptr_t)63);
-  v.ents = ents.data();
+  v.ents = (Entry*)((q + 63) & ~(uintptr_t)63);
   v.mask = (u32)(buckets - 1);
   return true;
 }
@@ -248,11 +250,22 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
   std::atomic<uint64_t> nodes{0}, probes{0}, hits{0}, ent_reads{0}, stores{0};
   std::atomic<int> failed{0};
   auto t0 = std::chrono::steady_clock::now();
+  // The shared-TT build gives all threads one TT as large as their private TTs together
+  // (threads rounded up to a power of two; reference cpu/cpu_main.cpp:44-51).
+  int shared_log2 = tt_log2;
+  while (DDX_SHARED_TT && (1 << (shared_log2 - tt_log2)) < threads) shared_log2++;
+  std::vector<u32> shared_tags;
+  std::vector<Entry> shared_ents;
+  TTView shared_view{};
+  if (DDX_SHARED_TT && !make_tt(shared_log2, shared_tags, shared_ents, shared_view)) {
+    ddx_set_error("ddx_cpu_solve_batch: cannot allocate a TT of 2^%d buckets", shared_log2);
+    return DDX_E_NOMEM;
+  }
   auto worker = [&](bool record) {
     std::vector<u32> tags;
     std::vector<Entry> ents;
-    TTView v;
-    if (!make_tt(tt_log2, tags, ents, v)) {
+    TTView v = shared_view;
+    if (!DDX_SHARED_TT && !make_tt(tt_log2, tags, ents, v)) {
       failed = 1;
       return;
     }

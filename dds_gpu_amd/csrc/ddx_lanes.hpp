@@ -37,6 +37,9 @@ struct GpuStack {
 #endif
 };
 
+// Largest TT_LOG2: per search, or, in the shared-TT build, of the one TT (2^28 buckets = 155 GB).
+constexpr int kMaxTTLog2 = DDX_SHARED_TT ? 28 : 20;
+
 struct LaneStore {
   u32* tags = nullptr;
   Entry* ents = nullptr;
@@ -60,8 +63,13 @@ struct LaneView {
 __device__ __forceinline__ void lane_attach(const LaneView& v, long long lane, Lane& L, GpuStack& st) {
   const size_t buckets = (size_t)1 << v.tt_log2;
   TTView tt;
+#if DDX_SHARED_TT
+  tt.tags = v.tags;  // one TT of 2^tt_log2 buckets serves every search
+  tt.ents = v.ents;
+#else
   tt.tags = v.tags + (size_t)lane * buckets * kWays;
   tt.ents = (Entry*)((char*)v.ents + (size_t)lane * buckets * kBucketEntryBytes);
+#endif
   tt.mask = (u32)(buckets - 1);
 #if DDX_FRAMES_DEPTH_MAJOR
   st.base = v.stacks + lane;

@@ -248,16 +248,17 @@ extern "C" int ddx_stream_open(const ddx_gpu_config* cfg_in, int ring_slots, ddx
   ddx_gpu_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ddx_gpu_default_config(&cfg);
-  if (DDX_TT_SIG && cfg.team != 1) {
-    ddx_set_error("ddx_stream_open: the signature-entry build runs one lane per search (team 1)");
+  if ((DDX_TT_SIG || DDX_SHARED_TT) && cfg.team != 1) {
+    ddx_set_error("ddx_stream_open: the %s build runs one lane per search (team 1)",
+                  DDX_TT_SIG ? "signature-entry" : "shared-TT");
     return DDX_E_ARG;
   }
-  if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > 20 || cfg.lanes < 64 || cfg.lanes % 256 ||
+  if (cfg.tt_log2 < kMinLog2 || cfg.tt_log2 > kMaxTTLog2 || cfg.lanes < 64 || cfg.lanes % 256 ||
       (cfg.jobs_per_deal != 5 && cfg.jobs_per_deal != 20) || cfg.seed_mode < 0 || cfg.seed_mode > 3 ||
       (cfg.team != 1 && cfg.team != kWideTeam) || (cfg.waves_per_simd != 0 && cfg.waves_per_simd != 3) ||
       cfg.split_min_nodes != 0) {
-    ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, 20], "
-                  "jobs_per_deal 5 or 20; the stream kernel does not split searches)");
+    ddx_set_error("ddx_stream_open: bad config (lanes multiple of 256, TT_LOG2 in [9, %d], "
+                  "jobs_per_deal 5 or 20; the stream kernel does not split searches)", kMaxTTLog2);
     return DDX_E_ARG;
   }
   int ndev = 0;

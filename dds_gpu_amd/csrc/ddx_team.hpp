@@ -203,9 +203,9 @@ struct Team16 {
 template <int TEAM>
 struct TeamOf;
 
-// Team size of the wide kernels: 16, or 1 in the signature-entry build (Team16's TT code
-// reads and writes exact entries), whose open calls then reject team = 16.
-constexpr int kWideTeam = DDX_TT_SIG ? 1 : 16;
+// Team size of the wide kernels: 16, or 1 in the signature-entry and shared-TT builds (Team16's
+// TT code reads and writes exact entries), whose open calls then reject team = 16.
+constexpr int kWideTeam = DDX_TT_SIG || DDX_SHARED_TT ? 1 : 16;
 template <>
 struct TeamOf<1> {
   using Ops = SeqOps;

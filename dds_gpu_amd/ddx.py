@@ -11,8 +11,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libddx.so")
 # Build variants of the same C ABI: "" the default exact TT entries, "sig" 12-byte signature
-# entries (the reference's -DDD_TT_SIG build).
-LIB_PATHS = {"": LIB_PATH, "sig": os.path.join(_HERE, "lib", "libddx_sig.so")}
+# entries (the reference's -DDD_TT_SIG build), "shared" one lock-free TT for all searches (its
+# -DDD_SHARED_TT build).
+LIB_PATHS = {"": LIB_PATH, "sig": os.path.join(_HERE, "lib", "libddx_sig.so"),
+             "shared": os.path.join(_HERE, "lib", "libddx_shared.so")}
 
 SEED_PLAIN, SEED_STEP, SEED_CERT, SEED_CERTP = 0, 1, 2, 3
 SCHED_PRIORITY, SCHED_CHAIN, SCHED_SPLIT, SCHED_THREAD = 0, 1, 2, 3

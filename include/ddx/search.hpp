@@ -213,15 +213,24 @@ struct SeqOps {
 };
 
 // New generation for a new solve; after 63 solves every tag of the TT is cleared
-// (reference dd_next_gen, src/dd.h:864-874).
+// (reference dd_next_gen, src/dd.h:864-874). The shared TT has no generations: its entries stay
+// valid for every deal with the same trump.
 template <class X = SeqOps>
 DDX_FN void next_generation(Ctx& c) {
+  if (DDX_SHARED_TT) return;
   c.gen++;
   if (c.gen >= (1u << kGenBits)) {
     X::clear_tags(c.tt);
     c.gen = 1;
   }
 }
+
+// What a tag's generation field holds: the solve's generation, or, in the shared TT, the trump
+// suit + 2 (reference DD_KEY, src/dd.h:223-227).
+DDX_FN u32 tag_key(const Ctx& c, const Pos& p) { return DDX_SHARED_TT ? (u32)(p.trump + 2) : c.gen; }
+// The store's last input: the generation (a full tag line gives up other generations first), or,
+// in the shared TT, the node count (which picks the way to overwrite when none is empty).
+DDX_FN u32 store_aux(const Ctx& c) { return DDX_SHARED_TT ? (u32)c.nodes : c.gen; }
 
 // Compile-time switches used only by the tests' "plain" build: no TT, no quick tricks, no
 // later tricks, no second-hand quick tricks, no small-card skip (reference DD_NO_*,
@@ -269,7 +278,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
     return (owner(p.hand, top) & 1) == 0 ? 1 : 0;
   }
   Key key;
-  make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
+  make_key(p.hand, p.leader, p.left, tag_key(c, p), c.tt.mask, key);
   int hint = -1;
   if (!DDX_NO_TT && !tt_off(c)) {
     int hit = X::probe(c.tt, key, need, rel, hint, c.ent_reads);
@@ -293,7 +302,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (decided) {
     const bool val = ns_lead ? q != 0 : q == 0;
     if (!tt_off(c)) {
-      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, store_aux(c));
       c.stores++;
     }
     rel = wr;
@@ -306,7 +315,7 @@ DDX_FN int enter_lead(Ctx& c, const Pos& p, int target, Frame& f, u64& rel, int&
   if (!DDX_NO_LT && lt) {
     const bool val = !ns_lead;
     if (!tt_off(c)) {
-      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, c.gen);
+      X::store(c.tt, key, wr, val ? need : 0, val ? p.left : need - 1, -1, store_aux(c));
       c.stores++;
     }
     rel = wr;
@@ -429,11 +438,11 @@ DDX_FN u64 finish(Ctx& c, const Pos& p, const Frame& f) {
   }
   if (p.nplayed == 0 && !DDX_NO_TT && !tt_off(c)) {
     Key key;
-    make_key(p.hand, p.leader, p.left, c.gen, c.tt.mask, key);
+    make_key(p.hand, p.leader, p.left, tag_key(c, p), c.tt.mask, key);
     int lb = f.lb, ub = f.ub;
     if (f.result) lb = f.need;
     else ub = f.need - 1;
-    X::store(c.tt, key, acc, lb, ub, f.best, c.gen);
+    X::store(c.tt, key, acc, lb, ub, f.best, store_aux(c));
     c.stores++;
   }
   return acc;

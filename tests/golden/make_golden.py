@@ -27,6 +27,9 @@ Fixtures:
                    oracle/_ref/ref_harness_sig), TT_LOG2 10, one context per group, jobs in
                    index order: tables and job nodes of the first 400 C1 endings, the first
                    100 11-card mixed endings and 12 full deals of full_2000.npz.
+  shared_tt.npz    the same deals and layout from the reference built with its lock-free
+                   shared TT (-DDD_SHARED_TT, oracle/_ref/ref_harness_shared): one TT of 2^10
+                   buckets per group, kept across the group's deals.
 """
 import argparse
 import os
@@ -119,17 +122,25 @@ def make_mixed(ref):
                         seed=MIXED_SEED, tt_log2=10)
 
 
-def make_sig(ref):
+def make_variant(ref, exe, fixture):
     groups = {"c1": deals(C1_SEED, 0, SIG_C1_N, C1_K),
               "k11": deals(MIXED_SEED + 11, 0, SIG_K11_N, 11),
               "full": deals(C2_SEED, 0, FULL_N, 13)[list(SIG_FULL)]}
     with ThreadPoolExecutor(max_workers=len(groups)) as ex:
-        res = dict(zip(groups, ex.map(lambda h: ref_tables(ref, h, 10, exe="ref_harness_sig"), groups.values())))
+        res = dict(zip(groups, ex.map(lambda h: ref_tables(ref, h, 10, exe=exe), groups.values())))
     out = {}
     for name, h in groups.items():
         out[name + "_hands"] = h
         out[name + "_tables"], out[name + "_nodes"] = res[name]
-    np.savez_compressed(os.path.join(HERE, "sig_entries.npz"), full_rows=np.array(SIG_FULL), tt_log2=10, **out)
+    np.savez_compressed(os.path.join(HERE, fixture), full_rows=np.array(SIG_FULL), tt_log2=10, **out)
+
+
+def make_sig(ref):
+    make_variant(ref, "ref_harness_sig", "sig_entries.npz")
+
+
+def make_shared(ref):
+    make_variant(ref, "ref_harness_shared", "shared_tt.npz")
 
 
 def make_funcs(ref):
@@ -152,7 +163,7 @@ def main():
     a = ap.parse_args()
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "REF=" + a.ref])
     jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full, "mixed": make_mixed,
-            "sig": make_sig}
+            "sig": make_sig, "shared": make_shared}
     for name, fn in jobs.items():
         if a.only and name not in a.only.split(","):
             continue

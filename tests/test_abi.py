@@ -24,9 +24,10 @@ def test_header_declares_the_boundary():
         assert must in names, must
 
 
-@pytest.mark.parametrize("variant", ["", "sig"])
+@pytest.mark.parametrize("variant", ["", "sig", "shared"])
 def test_library_exports_every_declared_symbol(ddx, variant):
-    """Both builds (exact TT entries, and 12-byte signature entries) export the whole ABI."""
+    """Every build (exact TT entries, 12-byte signature entries, the shared TT) exports the whole
+    ABI."""
     lib = ddx.lib(variant)
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
