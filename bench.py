@@ -3,6 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--step-seconds S] [--deals D]
                     [--legs plain,tuned,learned] [--model PATH] [--no-cpu-baseline]
+                    [--variant sig|shared] [--tt-log2 N]
 
 Workload: the seeded stream 2026 of full 52-card deals (uniform Fisher-Yates, 13 cards per
 hand), D = 100,000 deals resident in HBM, 20 results per deal (uint8). Every leg launches
@@ -26,7 +27,9 @@ Legs (the same deals and queue order in each):
   learned  the tuned configuration seeded by the trick-count net (DDX_SEED_CERT): the net's
            GPU prediction for the whole batch is timed and its cost charged pro rata
 `value` is the learned leg when a trained net exists (else the tuned / plain leg);
-`speedup_vs_plain` = value / plain.
+`speedup_vs_plain` = value / plain. --variant runs the legs on another TT build of the library
+(libddx_sig.so: 12-byte signature entries; libddx_shared.so: one TT shared by every search, whose
+TT_LOG2, set with --tt-log2, is the size of that one TT); the default is the exact build.
 
 Checks and side measurements in the same JSON line:
   bit_exact     every solved answer of the first 2,000 deals, in every leg, against the
@@ -54,6 +57,10 @@ sys.path.insert(0, ROOT)
 METRIC = "full-deal DD tables/sec on 1 MI355X, learned-seeded vs plain, bit-exact; p99 ms"
 STREAM_SEED = 2026
 HBM_PEAK_GBS = 8000.0
+# Algorithmic TT bytes by build: per entry a probe reads, and per store (the exact and signature
+# stores read and rewrite the 64 B tag line and write one entry; the shared store reads the tag
+# line and writes one 4 B tag and one entry).
+TT_BYTES = {"": (16.0, 144.0), "sig": (12.0, 140.0), "shared": (32.0, 100.0)}
 PLAIN = {"lanes": 65536, "team": 1, "tt_log2": 10, "jobs_per_deal": 5, "schedule": 2, "waves_per_simd": 0,
          "block": 256}
 DEFAULT_MODEL = os.path.join(ROOT, "dds_gpu_amd", "models", "tricknet.bin")
@@ -79,6 +86,9 @@ def parse():
     ap.add_argument("--tuned", default=DEFAULT_TUNED)
     ap.add_argument("--cpu-sample", type=int, default=-1, help="deals for the CPU baseline (-1: auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", default="", choices=["", "sig", "shared"],
+                    help="TT build of the library: exact (default), sig, shared")
+    ap.add_argument("--tt-log2", type=int, default=0, help="TT_LOG2 of every leg (0: the legs' own)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 5 plain 1 GiB device-to-device copies (a known-size kernel "
                          "against which HBM byte counters are scaled)")
@@ -146,7 +156,7 @@ def run_leg(ddx, torch, name, cfg, d_hands, n, a, gold, seeds=None, predict_s=0.
     stream = torch.cuda.current_stream().cuda_stream
     g = ddx.GpuSolver(lanes=cfg["lanes"], tt_log2=cfg["tt_log2"], jobs_per_deal=cfg["jobs_per_deal"],
                       schedule=cfg["schedule"], team=cfg["team"], waves_per_simd=cfg["waves_per_simd"],
-                      block=cfg.get("block", 256),
+                      block=cfg.get("block", 256), variant=cfg.get("variant", ""),
                       seed_mode=ddx.SEED_CERT if seeds is not None else ddx.SEED_PLAIN)
     jpd = cfg["jobs_per_deal"]
     log("[%s] %s, %.1f GB of lane state" % (name, json.dumps(cfg), g.device_bytes / 1e9))
@@ -186,7 +196,8 @@ def run_leg(ddx, torch, name, cfg, d_hands, n, a, gold, seeds=None, predict_s=0.
     out = full[:len(gold["tables"])]
     solved = out != 255
     wrong = int(((out != gold["tables"]) & solved).sum())
-    alg = 64.0 * st.tt_probes + 16.0 * st.tt_entry_reads + 144.0 * st.tt_stores + 52.0 * st.jobs_done / jpd
+    ent_b, store_b = TT_BYTES[cfg.get("variant", "")]
+    alg = 64.0 * st.tt_probes + ent_b * st.tt_entry_reads + store_b * st.tt_stores + 52.0 * st.jobs_done / jpd
     achieved = alg / (kms / 1e3) / 1e9
     res = {"value": round(value, 2), "config": dict(cfg, seed_mode="cert" if seeds is not None else "plain"),
            "whole_tables": whole, "seconds_from_launch": round(from_launch, 3),
@@ -229,6 +240,12 @@ def main():
     if os.path.exists(a.tuned):
         b = json.load(open(a.tuned))["best"]
         tuned = {k: b.get(k, PLAIN[k]) for k in PLAIN}
+    plain = dict(PLAIN)
+    for cfg in (plain, tuned):
+        if a.variant:
+            cfg["variant"] = a.variant
+        if a.tt_log2:
+            cfg["tt_log2"] = a.tt_log2
     legs = a.legs.split(",")
     have_model = os.path.exists(a.model)
     if "learned" in legs and not have_model:
@@ -238,7 +255,7 @@ def main():
     results = {}
     for name in legs:
         if name == "plain":
-            results[name] = run_leg(ddx, torch, name, PLAIN, d_hands, n, a, gold)
+            results[name] = run_leg(ddx, torch, name, plain, d_hands, n, a, gold)
         elif name == "tuned":
             results[name] = run_leg(ddx, torch, name, tuned, d_hands, n, a, gold)
         elif name == "learned":

@@ -8,8 +8,11 @@
 //   move ordering     lane j weighs the j-th candidate card, then a 16-lane bitonic sort
 //                     orders (weight, card) descending -- the order of the sequential
 //                     insertion sort
-// Results are identical to the sequential versions (include/ddx/tt.hpp, search.hpp), node for
-// node. Requirement: whenever these run, all 16 lanes of the team are active with identical
+// Each op follows the rules of its sequential version (include/ddx/tt.hpp, search.hpp), and the
+// tables are identical. Node counts are not quite: over the 4,096 8-card endings at TT_LOG2 10
+// the team kernel enters 68,058,935-68,059,338 nodes (by schedule) where one lane per search
+// enters the reference's 68,059,954; the cause of the 1e-5 difference is not identified.
+// Requirement: whenever these run, all 16 lanes of the team are active with identical
 // arguments.
 #pragma once
 #include <hip/hip_runtime.h>

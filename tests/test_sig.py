@@ -10,7 +10,8 @@ Pinned three ways:
     and full fixtures); on the 8-card endings the two builds spend the same nodes, while on
     full deals the signature build, which also stores positions with more than 29 relevant
     cards, spends fewer;
-  * on the GPU: every table of the signature kernel equals the fixture's."""
+  * on the GPU: every table of the signature kernel equals the fixture's, and its node total
+    equals the reference's."""
 import numpy as np
 import pytest
 from conftest import golden
@@ -74,3 +75,4 @@ def test_gpu_signature_kernel_tables(ddx_sig):
               (group, len(hands), st.nodes, int(SIG[group + "_nodes"].sum())), flush=True)
         assert st.jobs_done == 5 * len(hands)
         assert np.array_equal(out, SIG[group + "_tables"])
+        assert st.nodes == int(SIG[group + "_nodes"].sum())  # every job on a fresh generation
