@@ -522,32 +522,21 @@ DDX_FN SharedEntry* shared_bucket(const TTView& tt, u32 b) {
   return (SharedEntry*)((char*)tt.ents + (size_t)b * kBucketEntryBytes);
 }
 
-// Racing accesses: relaxed atomics on the host (threads share the TT), plain accesses on the GPU
-// (each word is read once per probe; stale or torn words fail the key and checksum checks).
-DDX_FN u64 sh_load(const u64* p) {
+// Racing accesses are relaxed atomics (agent scope on the GPU): each word is read exactly once,
+// so the checksum a reader checks covers the very words it then uses; stale or torn entries
+// fail the key and checksum checks.
+template <class T>
+DDX_FN T sh_load(const T* p) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  return *p;
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #else
   return __atomic_load_n(p, __ATOMIC_RELAXED);
 #endif
 }
-DDX_FN void sh_store(u64* p, u64 v) {
+template <class T>
+DDX_FN void sh_store(T* p, T v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  *p = v;
-#else
-  __atomic_store_n(p, v, __ATOMIC_RELAXED);
-#endif
-}
-DDX_FN u32 sh_load32(const u32* p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return *p;
-#else
-  return __atomic_load_n(p, __ATOMIC_RELAXED);
-#endif
-}
-DDX_FN void sh_store32(u32* p, u32 v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  *p = v;
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #else
   __atomic_store_n(p, v, __ATOMIC_RELAXED);
 #endif
@@ -582,7 +571,7 @@ DDX_FN int tt_probe(const TTView& tt, const Key& k, int need, u64& rel, int& hin
   const SharedEntry* base = shared_bucket(tt, k.bucket);
   const u64 who = (u64)(k.tag & 3) << 54 | (u64)((k.tag >> 2) & 63) << 56;
   for (int i = 0; i < kWays; i++) {
-    if (sh_load32(tags + i) != k.tag) continue;
+    if (sh_load(tags + i) != k.tag) continue;
     u64 w0, w1, w2, w3;
     ent_reads++;
     if (!shared_read(base + i, w0, w1, w2, w3)) continue;
@@ -627,7 +616,7 @@ DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, in
   SharedEntry* base = shared_bucket(tt, k.bucket);
   int victim = -1;
   for (int i = 0; i < kWays; i++) {
-    const u32 t = sh_load32(tags + i);
+    const u32 t = sh_load(tags + i);
     if (t == 0 && victim < 0) victim = i;
     if (t != k.tag) continue;
     u64 e0, e1, e2, e3;
@@ -643,12 +632,12 @@ DDX_FN void tt_store(const TTView& tt, const Key& k, u64 rel, int lb, int ub, in
   if (victim < 0) victim = (int)((nodes ^ (nodes >> 7)) & (kWays - 1));
   const u64 m = shared_meta(lb, ub, best);
   SharedEntry* e = base + victim;
-  sh_store32(tags + victim, 0);
+  sh_store(tags + victim, 0u);
   sh_store(&e->w0, w0);
   sh_store(&e->w1, w1);
   sh_store(&e->w2, k.lens);
   sh_store(&e->w3, m | shared_check(w0, w1, k.lens, m));
-  sh_store32(tags + victim, k.tag);
+  sh_store(tags + victim, k.tag);
 }
 #endif  // DDX_SHARED_TT
 
