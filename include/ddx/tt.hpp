@@ -543,7 +543,7 @@ DDX_FN void sh_store(T* p, T v) {
 }
 
 DDX_FN u64 shared_check(u64 w0, u64 w1, u64 w2, u64 w3) {
-  const u64 h = fmix64(w0 ^ fmix64(w1 ^ fmix64(w2 + 0x8CB92BA72F3D8DD7ull) + (w3 & 0x3FFF)));
+  const u64 h = fmix64(w0 ^ fmix64(w1 ^ (fmix64(w2 + 0x8CB92BA72F3D8DD7ull) + (w3 & 0x3FFF))));
   return h & 0xFFFFFFFF00000000ull;
 }
 

@@ -11,9 +11,12 @@ Pinned three ways:
   * under contention: CPU threads sharing one TT, and on the GPU 65,536 lanes sharing one small
     TT (stores overwrite each other's ways constantly, and readers meet entries being written),
     give every table of the fixtures."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
-from conftest import golden
+from conftest import ROOT, golden
 
 SH = golden("shared_tt.npz")
 
@@ -57,6 +60,19 @@ def test_entries_serve_later_deals(ddx_shared):
         spent.append(c.nodes - before)
     c.close()
     assert spent[1] < spent[0] / 2
+
+
+def test_torn_entries_are_misses(tmp_path):
+    """A stored entry answers; with one of its words taken from another entry (a read that met
+    a half-written way), or its bounds changed behind the checksum, it is a miss
+    (tests/support/shared_tt_check.cpp)."""
+    exe = str(tmp_path / "shared_tt_check")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wno-unknown-pragmas", "-DDDX_SHARED_TT=1",
+                           "-I" + os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "support", "shared_tt_check.cpp"), "-o", exe])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout.strip())
+    assert r.returncode == 0, r.stdout
 
 
 def test_cpu_threads_share_one_tt(ddx_shared):
