@@ -142,3 +142,34 @@ def test_dd_stream_records():
         assert 0 <= idx < len(hands) and idx not in seen
         seen.add(idx)
         assert np.array_equal(table, g["tables"][EASY_FULL[idx]]), idx
+
+
+# ---- the other TT builds' tools (bin/sig, bin/shared) ---------------------------------------
+
+VARIANT_TT = {"sig": "12", "shared": "16"}  # shared: TT_LOG2 is the size of the one TT
+
+
+@pytest.mark.parametrize("variant", ["sig", "shared"])
+def test_variant_dd_cpu(easy_file, variant):
+    good, bad = easy_file
+    r = run([tool(os.path.join(variant, "dd_cpu")), good, "0", "8", "4", VARIANT_TT[variant]])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert DD_CPU_LINE.match(r.stdout.decode().strip().splitlines()[-1]).groups() == ("8", "4", "0", "160")
+    assert run([tool(os.path.join(variant, "dd_cpu")), bad, "0", "8", "4", VARIANT_TT[variant]]).returncode == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["sig", "shared"])
+def test_variant_dd_gpu_and_stream(easy_file, variant):
+    good, _ = easy_file
+    r = run([tool(os.path.join(variant, "dd_gpu")), good, "0", "8", VARIANT_TT[variant]], env={"THREADS": "256"})
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    assert r.stdout.decode().strip().splitlines()[-1] == "0 wrong of 160 checked"
+    g = golden("full_2000.npz")
+    stdin = b"".join(struct.pack("<4Q", *[int(x) for x in h]) for h in g["hands"][EASY_FULL])
+    r = run([tool(os.path.join(variant, "dd_stream"))], env={"THREADS": "4096"}, stdin=stdin)
+    assert r.returncode == 0, r.stderr.decode()
+    assert len(r.stdout) == 28 * len(EASY_FULL)
+    for k in range(len(EASY_FULL)):
+        idx = struct.unpack_from("<q", r.stdout, 28 * k)[0]
+        assert np.array_equal(np.frombuffer(r.stdout, np.uint8, 20, 28 * k + 8), g["tables"][EASY_FULL[idx]])
