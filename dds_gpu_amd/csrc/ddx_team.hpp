@@ -11,8 +11,10 @@
 // Each op follows the rules of its sequential version (include/ddx/tt.hpp, search.hpp), and the
 // tables are identical. Node counts are not quite: over the 4,096 8-card endings at TT_LOG2 10
 // the team kernel enters 68,056,658-68,059,338 nodes, different from run to run and always a
-// little fewer than the 68,059,954 that one lane per search enters (the reference's count); the
-// cause of this nondeterminism is not identified.
+// little fewer than the 68,059,954 that one lane per search enters (the reference's count).
+// These ops restated lane by lane on the host (tests/support/team_emu.cpp) enter exactly the
+// reference's nodes, so the variation comes from how the kernel executes, not from the ops'
+// logic; its cause is not identified.
 // Requirement: whenever these run, all 16 lanes of the team are active with identical
 // arguments.
 #pragma once
