@@ -1,7 +1,7 @@
 """The 16-lane team ops of the GPU kernel (dds_gpu_amd/csrc/ddx_team.hpp), restated lane by lane
 on the host (tests/support/team_emu.cpp) and run through the same machine as the sequential ops:
 
-  * their logic is the sequential ops': every job of the C1 endings enters exactly the
+  * their logic is the sequential ops': every job of the first 256 C1 endings enters exactly the
     reference's nodes;
   * a job's nodes do not depend on what the search ran before (each job starts on a fresh TT
     generation): with the jobs run last-first, every job still enters the reference's nodes,
