@@ -29,6 +29,10 @@
 #include <stdio.h>
 #endif
 
+#ifndef DDX_PER_LEADER_TT
+#define DDX_PER_LEADER_TT 0
+#endif
+
 namespace ddx {
 
 enum { ST_JOB, ST_ROOT, ST_ENTER, ST_NEXT, ST_RET, ST_EXIT, ST_WAIT, ST_HELP, ST_COUNT };
@@ -178,6 +182,9 @@ DDX_FN void lane_search_done(Lane& L, const Batch& B) {
   L.decl++;
   if (L.decl <= L.last_decl) {
     L.prev_ns = ns < 1 ? 1 : ns;
+    // Measurement switch: the leads of a strain share no TT entries (reference
+    // -DDD_PER_LEADER_TT in dd_solve_strain, src/dd.h:932-934).
+    if (DDX_PER_LEADER_TT && B.seed_mode == SEED_PLAIN) next_generation(L.c);
     L.stage = ST_ROOT;
     return;
   }

@@ -30,6 +30,9 @@ Fixtures:
   shared_tt.npz    the same deals and layout from the reference built with its lock-free
                    shared TT (-DDD_SHARED_TT, oracle/_ref/ref_harness_shared): one TT of 2^10
                    buckets per group, kept across the group's deals.
+  per_leader_tt.npz  the first 400 C1 endings solved by the reference with its measurement
+                   switch -DDD_PER_LEADER_TT (no TT entries shared between the leads of a
+                   strain; oracle/_ref/ref_harness_perlead), TT_LOG2 10: tables and job nodes.
 """
 import argparse
 import os
@@ -143,6 +146,12 @@ def make_shared(ref):
     make_variant(ref, "ref_harness_shared", "shared_tt.npz")
 
 
+def make_perlead(ref):
+    h = deals(C1_SEED, 0, SIG_C1_N, C1_K)
+    t, nodes = ref_tables(ref, h, 10, exe="ref_harness_perlead")
+    np.savez_compressed(os.path.join(HERE, "per_leader_tt.npz"), hands=h, tables=t, nodes=nodes, tt_log2=10)
+
+
 def make_funcs(ref):
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "f.bin")
@@ -163,7 +172,7 @@ def main():
     a = ap.parse_args()
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref", "REF=" + a.ref])
     jobs = {"c1": make_c1, "funcs": make_funcs, "dds": make_dds, "full": make_full, "mixed": make_mixed,
-            "sig": make_sig, "shared": make_shared}
+            "sig": make_sig, "shared": make_shared, "perlead": make_perlead}
     for name, fn in jobs.items():
         if a.only and name not in a.only.split(","):
             continue

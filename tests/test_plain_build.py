@@ -43,3 +43,23 @@ def test_full_equals_plain(ddx, oracle, builds, K, n):
     assert np.array_equal(full, plain)
     assert np.array_equal(full, oracle.tables(hands))
     assert np_ > nf  # the pruning is switched off: the plain search enters more nodes
+
+
+def test_per_leader_switch_matches_the_reference(tmp_path):
+    """DDX_PER_LEADER_TT (the reference's -DDD_PER_LEADER_TT measurement switch: the four leads
+    of a strain share no TT entries) enters exactly the reference's nodes with the same switch
+    on the first 400 C1 endings (tests/golden/per_leader_tt.npz), 1.85x the shared-TT count."""
+    from conftest import golden
+    g = golden("per_leader_tt.npz")
+    out_so = str(tmp_path / "libperlead.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wno-unknown-pragmas",
+                           "-I" + os.path.join(ROOT, "include"), "-DDDX_PER_LEADER_TT=1", SRC, "-o", out_so])
+    lib = ctypes.CDLL(out_so)
+    lib.ddx_plain_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+    lib.ddx_plain_run.restype = ctypes.c_uint64
+    hands = np.ascontiguousarray(g["hands"], np.uint64)
+    out = np.zeros((len(hands), 20), np.uint8)
+    nodes = lib.ddx_plain_run(hands.ctypes.data, len(hands), 10, out.ctypes.data)
+    assert np.array_equal(out, g["tables"])
+    assert nodes == int(g["nodes"].sum())
+    assert nodes > 1.5 * int(golden("c1_endings.npz")["nodes"][:len(hands)].sum())
