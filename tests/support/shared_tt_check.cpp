@@ -1,4 +1,4 @@
-// Host check of the shared TT's torn-entry protection (include/ddx/tt.hpp, DDX_SHARED_TT): an
+// Host check of the shared TT's torn-entry protection (include/ddx/tt_shared.hpp, DDX_SHARED_TT): an
 // entry stored and probed back answers; the same entry with any one of its four words replaced
 // by the word of another entry (what a reader sees when it meets a half-written way), or with
 // its bounds changed behind the checksum, is a miss. Built and run by tests/test_shared_tt.py:
