@@ -253,7 +253,7 @@ extern "C" int ddx_cpu_solve_batch(const uint64_t* hands, int64_t n, int threads
   // The shared-TT build gives all threads one TT as large as their private TTs together
   // (threads rounded up to a power of two; reference cpu/cpu_main.cpp:44-51).
   int shared_log2 = tt_log2;
-  while (DDX_SHARED_TT && (1 << (shared_log2 - tt_log2)) < threads) shared_log2++;
+  while (DDX_SHARED_TT && shared_log2 < 30 && (1 << (shared_log2 - tt_log2)) < threads) shared_log2++;
   std::vector<u32> shared_tags;
   std::vector<Entry> shared_ents;
   TTView shared_view{};

@@ -52,6 +52,8 @@ def lib(variant=""):
     """Loads libddx.so, or a build variant of it (built by dds_gpu_amd.build); raises if it is
     missing."""
     if variant not in _libs:
+        if variant not in LIB_PATHS:
+            raise ValueError("unknown library build %r (one of %s)" % (variant, sorted(LIB_PATHS)))
         path = LIB_PATHS[variant]
         if not os.path.exists(path):
             raise RuntimeError("%s is not built: run `python -m dds_gpu_amd.build` "
