@@ -1,6 +1,10 @@
-"""The search machine reads no memory it did not write: built with the memory sanitizer (host
-code only, tests/support/msan_machine.cpp), with the TT entries and the frame stack left
-uninitialized, it solves 200 C1 endings with no report and enters the reference's nodes."""
+"""Host-code sanitizers (CPU only):
+
+  * the search machine reads no memory it did not write: built with the memory sanitizer
+    (tests/support/msan_machine.cpp), with the TT entries and the frame stack left
+    uninitialized, it solves 200 C1 endings with no report and enters the reference's nodes;
+  * the shared TT has no data race: four threads on one TT, built with the thread sanitizer
+    (tests/support/tsan_shared_tt.cpp), with no report and the reference's tables."""
 import os
 import shutil
 import subprocess
@@ -27,3 +31,18 @@ def test_machine_under_memory_sanitizer(tmp_path):
     nodes, checksum = map(int, r.stdout.split())
     assert nodes == int(c1["nodes"][:n].sum())
     assert checksum == int(c1["tables"][:n].astype(np.int64).sum())
+
+
+def test_threads_on_one_tt_under_thread_sanitizer(tmp_path):
+    """Four threads on one shared TT, built with the thread sanitizer (host code only,
+    tests/support/tsan_shared_tt.cpp): no race reported, and the tables are the reference's."""
+    exe = str(tmp_path / "tsan_shared_tt")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=thread", "-Wno-unknown-pragmas",
+                           "-DDDX_SHARED_TT=1", "-I" + os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "support", "tsan_shared_tt.cpp"), "-o", exe, "-pthread"])
+    c1 = golden("c1_endings.npz")
+    hp = str(tmp_path / "hands.bin")
+    np.ascontiguousarray(c1["hands"][:200], np.uint64).tofile(hp)
+    r = subprocess.run([exe, hp, "200"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ThreadSanitizer" not in r.stderr, r.stderr[-3000:]
+    assert r.stdout.split() == ["sum", str(int(c1["tables"][:200].astype(np.int64).sum()))]
