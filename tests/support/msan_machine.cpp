@@ -21,7 +21,7 @@ int main(int argc, char** argv) {
   const size_t buckets = (size_t)1 << 10;
   u32* tags = (u32*)aligned_alloc(64, buckets * kWays * sizeof(u32));
   for (size_t i = 0; i < buckets * kWays; i++) tags[i] = 0;
-  Entry* ents = (Entry*)aligned_alloc(64, buckets * kWays * sizeof(Entry));
+  Entry* ents = (Entry*)aligned_alloc(64, buckets * kBucketEntryBytes);  // the build's entry format
   TTView v{tags, ents, (u32)(buckets - 1)};
   uint8_t* out = (uint8_t*)malloc((size_t)n * 20);
   Batch B{hands, out, nullptr, n * 5, 5, SEED_PLAIN};

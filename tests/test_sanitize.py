@@ -4,7 +4,8 @@
     (tests/support/msan_machine.cpp), with the TT entries and the frame stack left
     uninitialized, it solves 200 C1 endings with no report and enters the reference's nodes;
   * the shared TT has no data race: four threads on one TT, built with the thread sanitizer
-    (tests/support/tsan_shared_tt.cpp), with no report and the reference's tables."""
+    (tests/support/tsan_shared_tt.cpp), with no report and the reference's tables;
+  * no undefined behaviour in any TT build's search (-fsanitize=undefined)."""
 import os
 import shutil
 import subprocess
@@ -46,3 +47,21 @@ def test_threads_on_one_tt_under_thread_sanitizer(tmp_path):
     r = subprocess.run([exe, hp, "200"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "ThreadSanitizer" not in r.stderr, r.stderr[-3000:]
     assert r.stdout.split() == ["sum", str(int(c1["tables"][:200].astype(np.int64).sum()))]
+
+
+@pytest.mark.parametrize("build", ["", "-DDDX_TT_SIG=1", "-DDDX_SHARED_TT=1"])
+def test_machine_under_undefined_behaviour_sanitizer(tmp_path, build):
+    """No undefined behaviour (shifts, overflows, misaligned accesses) in any TT build's search
+    on 200 C1 endings and 4 full deals (g++ -fsanitize=undefined, aborting on the first report)."""
+    exe = str(tmp_path / "ubsan_machine")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=undefined", "-fno-sanitize-recover=undefined",
+                           "-Wno-unknown-pragmas", "-I" + os.path.join(ROOT, "include")] + ([build] if build else []) +
+                          [os.path.join(ROOT, "tests", "support", "msan_machine.cpp"), "-o", exe])
+    c1, full = golden("c1_endings.npz"), golden("full_2000.npz")
+    rows = [1386, 707, 1081, 1897]
+    for hands, tables in ((c1["hands"][:200], c1["tables"][:200]), (full["hands"][rows], full["tables"][rows])):
+        hp = str(tmp_path / "hands.bin")
+        np.ascontiguousarray(hands, np.uint64).tofile(hp)
+        r = subprocess.run([exe, hp, str(len(hands))], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "runtime error" not in r.stderr, r.stderr[-3000:]
+        assert int(r.stdout.split()[1]) == int(tables.astype(np.int64).sum())
