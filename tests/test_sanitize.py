@@ -5,7 +5,8 @@
     uninitialized, it solves 200 C1 endings with no report and enters the reference's nodes;
   * the shared TT has no data race: four threads on one TT, built with the thread sanitizer
     (tests/support/tsan_shared_tt.cpp), with no report and the reference's tables;
-  * no undefined behaviour in any TT build's search (-fsanitize=undefined)."""
+  * no undefined behaviour in any TT build's search (-fsanitize=undefined);
+  * the host side of the C ABI runs clean under the address sanitizer."""
 import os
 import shutil
 import subprocess
@@ -65,3 +66,18 @@ def test_machine_under_undefined_behaviour_sanitizer(tmp_path, build):
         r = subprocess.run([exe, hp, str(len(hands))], capture_output=True, text=True, timeout=300)
         assert r.returncode == 0 and "runtime error" not in r.stderr, r.stderr[-3000:]
         assert int(r.stdout.split()[1]) == int(tables.astype(np.int64).sum())
+
+
+def test_host_abi_under_address_sanitizer(tmp_path):
+    """The host side of the C ABI (tests/support/asan_abi.cpp with dds_gpu_amd/csrc/ddx_host.cpp,
+    g++ -fsanitize=address,undefined): CPU batch solves with one and four threads, strain
+    solves, a Pgx round trip and argument errors, with no report and no leak."""
+    exe = str(tmp_path / "asan_abi")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-Wno-unknown-pragmas",
+                           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "dds_gpu_amd", "csrc"),
+                           os.path.join(ROOT, "tests", "support", "asan_abi.cpp"),
+                           os.path.join(ROOT, "dds_gpu_amd", "csrc", "ddx_host.cpp"), "-o", exe, "-pthread"])
+    r = subprocess.run([exe, str(tmp_path / "t.npy"), str(tmp_path / "missing.npy")], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), (r.returncode, r.stderr[-3000:])
